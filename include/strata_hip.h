@@ -1,0 +1,231 @@
+/*
+ * strata_hip.h — C ABI of the MI355X (gfx950) raster stencil library.
+ *
+ * The seam is the reference engine's two entry points, which every
+ * terrain.*, focal.* and transfer.* function funnels into:
+ *
+ *   exec.ProcessN(ctx, dst, src []raster.Float32Raster, k Kernel, opts)
+ *       internal/exec/process.go:28           -> strata_process
+ *   exec.ProcessChunked(ctx, dst []RasterSink, src []RasterSource, k, opts)
+ *       internal/exec/chunked.go:33           -> strata_process_chunked
+ *
+ * The Kernel value is replaced by a plain descriptor (strata_kernel) that
+ * carries the constants the caller already resolved in float32
+ * (terrain/stencil.go:36-47 cellSizes, terrain/slope.go:84-94
+ * slopeScale, terrain/hillshade.go:94-117 hillshadeLight,
+ * terrain/curvature.go:113-127, terrain/fit.go:36-70,
+ * focal/correlate.go:70-88, focal/separable.go:53-89), so the library
+ * never restates Go's math.Sin/Cos/Mod/Exp.
+ *
+ * Plain pointers and sizes only. Rasters may live in device memory
+ * (STRATA_MEM_DEVICE, the fast path: inputs already resident in HBM) or
+ * host memory (STRATA_MEM_HOST: staged through pinned buffers). Only the
+ * outputs' own cells and validity bits are written (internal/exec/doc.go:
+ * 107-108); row padding and a parent's other bits are left alone.
+ *
+ * Programming errors (the reference's panics) return STRATA_EINVAL
+ * before anything is written; strata_last_error() gives the message,
+ * prefixed like the reference's ("engine: ...").
+ */
+#ifndef STRATA_HIP_H
+#define STRATA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define STRATA_OK 0
+#define STRATA_CANCELLED 1   /* ctx.Err(): a prefix of the plan was written */
+#define STRATA_EINVAL (-1)   /* programming error (the reference panics)  */
+#define STRATA_EDEVICE (-2)  /* HIP runtime or device failure             */
+#define STRATA_EIO (-3)      /* a source or sink callback failed          */
+#define STRATA_ENOMEM (-4)   /* device or pinned allocation failed        */
+
+/* ------------------------------------------------------------------ */
+/* Rasters: mirrors raster.Float32Raster (raster/raster.go:41-54)      */
+/* ------------------------------------------------------------------ */
+#define STRATA_MEM_DEVICE 0
+#define STRATA_MEM_HOST 1
+
+typedef struct strata_raster {
+    float *data;          /* cell (x, y) is data[y*stride + x]           */
+    int64_t width;
+    int64_t height;
+    int64_t stride;       /* >= width                                    */
+    uint64_t *valid;      /* NULL: every cell valid; else LSB-first bits  */
+    int64_t valid_offset; /* bit index of data[0] in valid                */
+    int32_t mem;          /* STRATA_MEM_DEVICE or STRATA_MEM_HOST         */
+    int32_t reserved;
+} strata_raster;
+
+/* ------------------------------------------------------------------ */
+/* Kernel descriptor: the reference's name-registered kernels          */
+/* (internal/opkernel/opkernel.go:32-52, terrain/opkernel.go:14-43,    */
+/* focal/opkernel.go:12-19)                                            */
+/* ------------------------------------------------------------------ */
+enum strata_op {
+    STRATA_OP_GRADIENT = 1,   /* terrain.Gradient: 1 input, 2 outputs (dx, dy) */
+    STRATA_OP_SLOPE = 2,      /* terrain.Slope                                 */
+    STRATA_OP_ASPECT = 3,     /* terrain.Aspect                                */
+    STRATA_OP_HILLSHADE = 4,  /* terrain.Hillshade                             */
+    STRATA_OP_CURVATURE = 5,  /* terrain.Curvature                             */
+    STRATA_OP_CORRELATE = 10, /* focal.Correlate / Convolve (weights pre-rotated) */
+    STRATA_OP_SEPARABLE = 11, /* focal.CorrelateSeparable                      */
+    STRATA_OP_MEAN = 12,      /* focal.Mean                                    */
+    STRATA_OP_MIN = 13,       /* focal.Min                                     */
+    STRATA_OP_MAX = 14,       /* focal.Max                                     */
+    STRATA_OP_LOOKUP = 20,    /* transfer.Lookup (radius 0)                    */
+    STRATA_OP_RECLASS = 21,   /* transfer.Reclass (radius 0)                   */
+};
+
+#define STRATA_FLAG_ATAN 0x1   /* slope: atan(m)*scale (degrees, radians)     */
+#define STRATA_FLAG_TRIG 0x2   /* aspect: trigonometric angle                 */
+#define STRATA_FLAG_FIT 0x100  /* terrain op through Wood's quadratic fit,    */
+                               /* radius = FitRadius (terrain/fit.go:187-226) */
+#define STRATA_CURV_SHIFT 4    /* curvature kind (0 profile, 1 plan, 2 mean)  */
+#define STRATA_FLAG_MFMA 0x200 /* correlate r>=4: split-BF16x3 MFMA path      */
+                               /* (approximate; see strata_process docs)     */
+
+/*
+ * k[] layout by op (all already-resolved float32 constants):
+ *   Horn ops (radius 1):  k0 kx, k1 ky; slope k2 scale; aspect k2 flat;
+ *                         hillshade k2 c, k3 bx, k4 by
+ *   ZT curvature:         k0..k4 = kp, kq, kr, kt, ks
+ *   FIT (any terrain op): k0..k4 = Wood factors kp, kq, kr, kt, ks;
+ *                         slope k5 scale; aspect k5 flat;
+ *                         hillshade k5 c, k6 bx, k7 by
+ * taps (host pointer, copied during the call):
+ *   CORRELATE: (2r+1)^2 weights, row-major; SEPARABLE: row[2r+1] then
+ *   col[2r+1]; LOOKUP: xs[n] then ys[n] (ntaps = 2n);
+ *   RECLASS: breaks[n] then values[n+1] (ntaps = 2n+1).
+ */
+typedef struct strata_kernel {
+    int32_t op;
+    int32_t radius;
+    int32_t flags;
+    int32_t ntaps;
+    float k[16];
+    const float *taps;
+} strata_kernel;
+
+/* ------------------------------------------------------------------ */
+/* Options: engine.Options (engine/engine.go:7-28) plus device plumbing */
+/* ------------------------------------------------------------------ */
+typedef struct strata_options {
+    int64_t tile_width;  /* 0: raster width                              */
+    int64_t tile_height; /* 0: raster height                             */
+    int64_t workers;     /* scheduling hint only; never changes results  */
+    void *stream;        /* hipStream_t to run on; NULL: the null stream  */
+    const volatile int32_t *cancel; /* host flag, nonzero = ctx done;
+                                       polled between units of the plan */
+    int32_t async;       /* 1: return once enqueued (device operands,
+                            cancel == NULL only); 0: return when done    */
+    int32_t reserved;
+} strata_options;
+
+/* engine.Stats (engine/stats.go:77-104), field for field, in the
+ * closed form the reference's band and tile loops produce. */
+typedef struct strata_stats {
+    int64_t cells;
+    int64_t ideal;
+    int64_t tiles;
+    int64_t bands;
+    int64_t source_read;
+    int64_t sink_written;
+    int64_t kernel_read;
+    int64_t kernel_written;
+} strata_stats;
+
+/* ------------------------------------------------------------------ */
+/* Library                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Selects the HIP device and loads the kernels. Optional: every entry
+ * point initialises the current device on first use. */
+int strata_init(int device);
+
+/* Message of the calling thread's last error, or "" after success. */
+const char *strata_last_error(void);
+
+/* Version string of the library build. */
+const char *strata_version(void);
+
+/*
+ * exec.ProcessN (internal/exec/process.go:28): run k over every cell of
+ * dst[0..ndst) from src[0..nsrc). Same checks (process.go:37-135, minus
+ * Go-slice overlap that a caller checks before crossing the boundary),
+ * same edge policy (NaN ring of width radius), same validity rules
+ * (internal/exec/halo.go:161-290) and the same Stats for every Options
+ * value (internal/exec/halo.go:32-39,103-104, worker.go:59-68).
+ * Cancellation: units of the plan (bands) are claimed in order and
+ * always finished, so after STRATA_CANCELLED the written bands are a
+ * prefix of the plan (internal/exec/doc.go:165-183).
+ * Returns STRATA_OK, STRATA_CANCELLED, or a negative error. Stats, when
+ * not NULL, are added to (engine/stats.go:63-70).
+ */
+int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src,
+                   int nsrc, const strata_options *opts, strata_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Chunked: engine.RasterSource / RasterSink (engine/source.go:19-71)  */
+/* ------------------------------------------------------------------ */
+
+/* ReadWindow (engine/source.go:40): fill the w x h cells of buf (host
+ * memory, row stride `stride`, validity at bit valid_offset of valid when
+ * valid != NULL) with the region at (x, y). Return 0, or nonzero for an
+ * IO error. */
+typedef int (*strata_read_fn)(void *user, float *buf, int64_t w, int64_t h, int64_t stride, uint64_t *valid,
+                              int64_t valid_offset, int64_t x, int64_t y);
+/* WriteWindow (engine/source.go:67). */
+typedef int (*strata_write_fn)(void *user, const float *buf, int64_t w, int64_t h, int64_t stride,
+                               const uint64_t *valid, int64_t valid_offset, int64_t x, int64_t y);
+
+/* A source is either resident (a raster the library reads windows of
+ * directly, MemorySource over device memory: engine/source.go:76-107) or
+ * a callback reader. */
+typedef struct strata_source {
+    int64_t width, height;
+    int32_t masked;          /* Masked() */
+    int32_t reserved;
+    strata_read_fn read;     /* used when resident is NULL */
+    void *user;
+    const strata_raster *resident;
+} strata_source;
+
+typedef struct strata_sink {
+    int64_t width, height;
+    int32_t masked;
+    int32_t reserved;
+    strata_write_fn write;   /* used when resident is NULL */
+    void *user;
+    const strata_raster *resident;
+} strata_sink;
+
+/*
+ * exec.ProcessChunked (internal/exec/chunked.go:33): tiles in row-major
+ * order, each read with its halo clipped to the raster, computed exactly
+ * as strata_process computes those cells, and written to every sink.
+ * Device-resident sources and sinks form a device tile queue (no copies);
+ * callback ones are staged through pinned host buffers with copies
+ * overlapped on separate streams. Tiles are claimed in order and finished
+ * once claimed, so a cancelled or failed call leaves a prefix of whole
+ * tiles (engine/doc.go:188-215). Stats as chunked.go:302-374.
+ */
+int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int ndst, const strata_source *src,
+                           int nsrc, const strata_options *opts, strata_stats *stats);
+
+/* Closed-form Stats of a call that completes (no device work): what
+ * strata_process (chunked = 0) or strata_process_chunked (chunked = 1)
+ * adds for a width x height raster. */
+int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
+                      const strata_options *opts, int chunked, strata_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

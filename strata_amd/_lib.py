@@ -1,0 +1,185 @@
+"""ctypes binding of include/strata_hip.h (libstrata_hip.so).
+
+The library is built in-tree by strata_amd/build.py. Nothing here falls
+back to another implementation: if the library is missing or fails to
+load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int32, c_int64, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstrata_hip.so")
+
+# Status codes
+OK = 0
+CANCELLED = 1
+EINVAL = -1
+EDEVICE = -2
+EIO = -3
+ENOMEM = -4
+
+MEM_DEVICE = 0
+MEM_HOST = 1
+
+# Ops (enum strata_op)
+OP_GRADIENT = 1
+OP_SLOPE = 2
+OP_ASPECT = 3
+OP_HILLSHADE = 4
+OP_CURVATURE = 5
+OP_CORRELATE = 10
+OP_SEPARABLE = 11
+OP_MEAN = 12
+OP_MIN = 13
+OP_MAX = 14
+OP_LOOKUP = 20
+OP_RECLASS = 21
+
+FLAG_ATAN = 0x1
+FLAG_TRIG = 0x2
+FLAG_FIT = 0x100
+CURV_SHIFT = 4
+FLAG_MFMA = 0x200
+
+MAX_TAPS = 320
+
+
+class Raster(ctypes.Structure):
+    _fields_ = [
+        ("data", c_void_p),
+        ("width", c_int64),
+        ("height", c_int64),
+        ("stride", c_int64),
+        ("valid", c_void_p),
+        ("valid_offset", c_int64),
+        ("mem", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [
+        ("op", c_int32),
+        ("radius", c_int32),
+        ("flags", c_int32),
+        ("ntaps", c_int32),
+        ("k", c_float * 16),
+        ("taps", c_void_p),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("tile_width", c_int64),
+        ("tile_height", c_int64),
+        ("workers", c_int64),
+        ("stream", c_void_p),
+        ("cancel", c_void_p),
+        ("async_", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("cells", c_int64),
+        ("ideal", c_int64),
+        ("tiles", c_int64),
+        ("bands", c_int64),
+        ("source_read", c_int64),
+        ("sink_written", c_int64),
+        ("kernel_read", c_int64),
+        ("kernel_written", c_int64),
+    ]
+
+
+READ_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                           c_int64)
+WRITE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                            c_int64)
+
+
+class Source(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int64),
+        ("height", c_int64),
+        ("masked", c_int32),
+        ("reserved", c_int32),
+        ("read", READ_FN),
+        ("user", c_void_p),
+        ("resident", POINTER(Raster)),
+    ]
+
+
+class Sink(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int64),
+        ("height", c_int64),
+        ("masked", c_int32),
+        ("reserved", c_int32),
+        ("write", WRITE_FN),
+        ("user", c_void_p),
+        ("resident", POINTER(Raster)),
+    ]
+
+
+# The symbols include/strata_hip.h declares, for the export check.
+EXPORTS = (
+    "strata_init",
+    "strata_last_error",
+    "strata_version",
+    "strata_process",
+    "strata_process_chunked",
+    "strata_plan_stats",
+)
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the library once. Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m strata_amd.build` "
+            "(the HIP library is the only implementation)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.strata_init.argtypes = [c_int]
+    L.strata_init.restype = c_int
+    L.strata_last_error.argtypes = []
+    L.strata_last_error.restype = c_char_p
+    L.strata_version.argtypes = []
+    L.strata_version.restype = c_char_p
+    L.strata_process.argtypes = [POINTER(Kernel), POINTER(Raster), c_int, POINTER(Raster), c_int,
+                                 POINTER(Options), POINTER(Stats)]
+    L.strata_process.restype = c_int
+    L.strata_process_chunked.argtypes = [POINTER(Kernel), POINTER(Sink), c_int, POINTER(Source), c_int,
+                                         POINTER(Options), POINTER(Stats)]
+    L.strata_process_chunked.restype = c_int
+    L.strata_plan_stats.argtypes = [POINTER(Kernel), c_int, c_int, c_int64, c_int64, POINTER(Options), c_int,
+                                    POINTER(Stats)]
+    L.strata_plan_stats.restype = c_int
+    _bind_optional(L)
+    _lib = L
+    return L
+
+
+def _bind_optional(L) -> None:
+    """Bind the batch and serving entry points when the build has them."""
+    for name, argtypes, restype in _OPTIONAL:
+        fn = getattr(L, name, None)
+        if fn is not None:
+            fn.argtypes = argtypes
+            fn.restype = restype
+
+
+_OPTIONAL: list = []
+
+
+def last_error() -> str:
+    return lib().strata_last_error().decode("utf-8", "replace")

@@ -1,0 +1,738 @@
+// runtime.hip — host side of the C ABI: checks, the plan, Stats, staging
+// of host-memory operands, the cancellation units and the chunked tile
+// queue. Device work is launched through kernels.hpp.
+//
+// Reference behaviour restated here:
+//   checks           internal/exec/process.go:37-135, chunked.go:42-104
+//   plan and bands   internal/exec/tile.go:298-359
+//   band accounting  internal/exec/halo.go:21-57,97-105; worker.go:59-68
+//   chunked tiles    internal/exec/chunked.go:159-374
+//   cancellation     internal/exec/worker.go:112-181 (claims in order,
+//                    a claimed unit always finishes: a prefix is written)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "strata_hip.h"
+
+using namespace strata;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(STRATA_EDEVICE, "strata: %s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                                                                                  \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess) return hip_fail(e_, what);                                                     \
+    } while (0)
+
+constexpr int64_t kBytesPerCell = 4;
+constexpr int64_t kBandCells = 1 << 16;  // internal/exec/tile.go:27
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Kernel descriptors
+// ---------------------------------------------------------------------------
+
+int n_outputs(const strata_kernel *k) { return k->op == STRATA_OP_GRADIENT ? 2 : 1; }
+
+int check_kernel(const strata_kernel *k, int ndst, int nsrc)
+{
+    if (!k) return fail(STRATA_EINVAL, "engine: nil kernel");
+    const bool fit = (k->flags & STRATA_FLAG_FIT) != 0;
+    switch (k->op) {
+    case STRATA_OP_GRADIENT:
+    case STRATA_OP_SLOPE:
+    case STRATA_OP_ASPECT:
+    case STRATA_OP_HILLSHADE:
+    case STRATA_OP_CURVATURE:
+        if (fit ? (k->radius < 1 || k->radius > kMaxRadius) : k->radius != 1)
+            return fail(STRATA_EINVAL, "terrain: FitRadius must be 0 to %d, got radius %d", kMaxRadius, k->radius);
+        if (k->op == STRATA_OP_CURVATURE && ((k->flags >> STRATA_CURV_SHIFT) & 3) > 2)
+            return fail(STRATA_EINVAL, "terrain: unknown CurvatureType %d", (k->flags >> STRATA_CURV_SHIFT) & 3);
+        break;
+    case STRATA_OP_CORRELATE:
+    case STRATA_OP_SEPARABLE:
+    case STRATA_OP_MEAN:
+    case STRATA_OP_MIN:
+    case STRATA_OP_MAX: {
+        if (k->radius < 1 || k->radius > kMaxRadius)
+            return fail(STRATA_EINVAL, "focal: Radius must be 1 to %d, got %d", kMaxRadius, k->radius);
+        const int n = 2 * k->radius + 1;
+        if (k->op == STRATA_OP_CORRELATE && k->ntaps != n * n)
+            return fail(STRATA_EINVAL, "focal: Radius %d needs %d×%d = %d Weights, got %d", k->radius, n, n, n * n,
+                        k->ntaps);
+        if (k->op == STRATA_OP_SEPARABLE && k->ntaps != 2 * n)
+            return fail(STRATA_EINVAL, "focal: Radius %d needs %d Row and %d Col taps, got %d in all", k->radius, n,
+                        n, k->ntaps);
+        break;
+    }
+    case STRATA_OP_LOOKUP:
+        if (k->radius != 0 || k->ntaps < 2 || (k->ntaps & 1))
+            return fail(STRATA_EINVAL, "transfer.Lookup: xs and ys must have equal, non-zero length");
+        break;
+    case STRATA_OP_RECLASS:
+        if (k->radius != 0 || k->ntaps < 1 || !(k->ntaps & 1))
+            return fail(STRATA_EINVAL, "transfer.Reclass: values must hold one more element than breaks");
+        break;
+    default:
+        return fail(STRATA_EINVAL, "engine: unknown kernel op %d", k->op);
+    }
+    if (k->ntaps < 0 || k->ntaps > kMaxTaps)
+        return fail(STRATA_EINVAL, "engine: %d taps exceed the device limit of %d", k->ntaps, kMaxTaps);
+    if (k->ntaps > 0 && !k->taps) return fail(STRATA_EINVAL, "engine: kernel has %d taps and a nil pointer", k->ntaps);
+    if (nsrc != 1 || ndst != n_outputs(k))
+        return fail(STRATA_EINVAL, "engine: kernel takes %d inputs and %d outputs, got %d and %d", 1, n_outputs(k),
+                    nsrc, ndst);
+    return STRATA_OK;
+}
+
+Params make_params(const strata_kernel *k)
+{
+    Params P;
+    memset(&P, 0, sizeof P);
+    P.op = k->op;
+    P.radius = k->radius;
+    P.flags = k->flags;
+    P.ntaps = k->ntaps;
+    memcpy(P.k, k->k, sizeof P.k);
+    if (k->ntaps > 0) memcpy(P.taps, k->taps, sizeof(float) * (size_t)k->ntaps);
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+// Rasters
+// ---------------------------------------------------------------------------
+
+int64_t data_len(const strata_raster &r) { return (r.height - 1) * r.stride + r.width; }
+
+// raster.Float32Raster.Validate (raster/raster.go:172-203), minus the
+// slice lengths, which the caller owns.
+int check_raster(const char *name, int i, const strata_raster &r)
+{
+    if (r.width <= 0 || r.height <= 0)
+        return fail(STRATA_EINVAL, "engine: %s[%d]: raster: dimensions must be positive, got %lld×%lld", name, i,
+                    (long long)r.width, (long long)r.height);
+    if (r.stride < r.width)
+        return fail(STRATA_EINVAL, "engine: %s[%d]: raster: stride %lld is less than width %lld", name, i,
+                    (long long)r.stride, (long long)r.width);
+    if (!r.data) return fail(STRATA_EINVAL, "engine: %s[%d]: nil data", name, i);
+    if (r.valid && r.valid_offset < 0)
+        return fail(STRATA_EINVAL, "engine: %s[%d]: raster: negative ValidOffset %lld", name, i,
+                    (long long)r.valid_offset);
+    if (r.mem != STRATA_MEM_DEVICE && r.mem != STRATA_MEM_HOST)
+        return fail(STRATA_EINVAL, "engine: %s[%d]: unknown memory kind %d", name, i, r.mem);
+    return STRATA_OK;
+}
+
+// A host-memory raster mirrored in device memory for the call. Device
+// memory holds the raster's whole span (cells and padding) and the mask
+// words its bits touch; only cells and own bits are copied back.
+struct Staged {
+    strata_raster dev{};
+    void *dbuf = nullptr;
+    uint64_t *dmask = nullptr;
+    int64_t w0 = 0, nwords = 0;
+    ~Staged()
+    {
+        if (dbuf) (void)hipFree(dbuf);
+        if (dmask) (void)hipFree(dmask);
+    }
+};
+
+int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
+{
+    s.dev = h;
+    s.dev.mem = STRATA_MEM_DEVICE;
+    const int64_t n = data_len(h);
+    HIP_TRY(hipMalloc(&s.dbuf, (size_t)n * sizeof(float)), "staging data");
+    s.dev.data = (float *)s.dbuf;
+    if (copy_data)
+        HIP_TRY(hipMemcpyAsync(s.dbuf, h.data, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st), "copy in");
+    if (h.valid) {
+        s.w0 = h.valid_offset >> 6;
+        const int64_t w1 = (h.valid_offset + n - 1) >> 6;
+        s.nwords = w1 - s.w0 + 1;
+        HIP_TRY(hipMalloc((void **)&s.dmask, (size_t)s.nwords * 8), "staging mask");
+        HIP_TRY(hipMemcpyAsync(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, hipMemcpyHostToDevice, st),
+                "copy mask in");
+        s.dev.valid = s.dmask;
+        s.dev.valid_offset = h.valid_offset - 64 * s.w0;
+    }
+    return STRATA_OK;
+}
+
+int stage_out(const strata_raster &h, Staged &s, hipStream_t st)
+{
+    HIP_TRY(hipMemcpy2DAsync(h.data, (size_t)h.stride * 4, s.dbuf, (size_t)h.stride * 4, (size_t)h.width * 4,
+                             (size_t)h.height, hipMemcpyDeviceToHost, st),
+            "copy out");
+    if (h.valid)
+        HIP_TRY(hipMemcpyAsync(h.valid + s.w0, s.dmask, (size_t)s.nwords * 8, hipMemcpyDeviceToHost, st),
+                "copy mask out");
+    return STRATA_OK;
+}
+
+SrcView src_view(const strata_raster &r, int64_t sx, int64_t sy, int64_t W, int64_t H)
+{
+    SrcView v;
+    v.p = r.data;
+    v.stride = r.stride;
+    v.sx = sx;
+    v.sy = sy;
+    v.lx0 = std::max<int64_t>(0, sx);
+    v.ly0 = std::max<int64_t>(0, sy);
+    v.lx1 = std::min<int64_t>(W, sx + r.width);
+    v.ly1 = std::min<int64_t>(H, sy + r.height);
+    return v;
+}
+
+MaskView mask_view(const strata_raster &r, int64_t x0, int64_t y0)
+{
+    MaskView m;
+    m.bits = r.valid;
+    m.off = r.valid_offset;
+    m.stride = r.stride;
+    m.x0 = x0;
+    m.y0 = y0;
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Plan (internal/exec/tile.go:298-359) and Stats (engine/stats.go)
+// ---------------------------------------------------------------------------
+
+struct Plan {
+    int64_t w = 0, h = 0, tileW = 0, tileH = 0, bandRows = 0, tilesX = 0, tileRows = 0, perTile = 0,
+            perLastTile = 0, bands = 0;
+};
+
+Plan new_plan(int64_t w, int64_t h, int64_t tileW, int64_t tileH)
+{
+    Plan p;
+    p.w = w;
+    p.h = h;
+    p.tileW = tileW > 0 ? std::min(tileW, w) : w;
+    p.tileH = tileH > 0 ? std::min(tileH, h) : h;
+    if (w == 0 || h == 0) return p;
+    p.bandRows = std::max<int64_t>(1, kBandCells / p.tileW);
+    p.tilesX = cdiv(w, p.tileW);
+    p.tileRows = cdiv(h, p.tileH);
+    p.perTile = cdiv(p.tileH, p.bandRows);
+    p.perLastTile = cdiv(h - (p.tileRows - 1) * p.tileH, p.bandRows);
+    p.bands = p.tilesX * ((p.tileRows - 1) * p.perTile + p.perLastTile);
+    return p;
+}
+
+struct Rect {
+    int64_t x0, y0, x1, y1;
+};
+
+Rect plan_band(const Plan &p, int64_t i)
+{
+    const int64_t perRow = p.tilesX * p.perTile;
+    int64_t tr = i / perRow, j = i % perRow, per = p.perTile;
+    if (tr >= p.tileRows - 1) {
+        tr = p.tileRows - 1;
+        j = i - (p.tileRows - 1) * perRow;
+        per = p.perLastTile;
+    }
+    const int64_t tx = j / per, b = j % per;
+    Rect r;
+    r.x0 = tx * p.tileW;
+    r.x1 = std::min(r.x0 + p.tileW, p.w);
+    const int64_t ty = tr * p.tileH;
+    r.y0 = ty + b * p.bandRows;
+    r.y1 = std::min(std::min(r.y0 + p.bandRows, ty + p.tileH), p.h);
+    return r;
+}
+
+// One band's counters (internal/exec/halo.go:22-39, 97-105), for the
+// band [x0, x1) x [y0, y1) of a W x H raster.
+void band_stats(strata_stats &s, int r, int64_t W, int64_t H, int nin, int nout, const Rect &b)
+{
+    const int64_t cells = (b.x1 - b.x0) * (b.y1 - b.y0);
+    const int64_t ix0 = std::max<int64_t>(b.x0, r), ix1 = std::min<int64_t>(b.x1, W - r);
+    const int64_t iy0 = std::max<int64_t>(b.y0, r), iy1 = std::min<int64_t>(b.y1, H - r);
+    const bool interior = ix0 < ix1 && iy0 < iy1;
+    const int64_t inner = interior ? (ix1 - ix0) * (iy1 - iy0) : 0;
+    s.cells += cells;
+    s.bands += 1;
+    s.kernel_written += (cells - inner) * kBytesPerCell * nout;
+    if (interior) {
+        s.kernel_read += (ix1 - ix0 + 2 * r) * (iy1 - iy0 + 2 * r) * kBytesPerCell * nin;
+        s.kernel_written += inner * kBytesPerCell * nout;
+    }
+}
+
+void add_stats(strata_stats *out, const strata_stats &s, int nin, int nout)
+{
+    if (!out) return;
+    out->cells += s.cells;
+    out->ideal += s.cells * kBytesPerCell * (nin + nout);
+    out->tiles += s.tiles;
+    out->bands += s.bands;
+    out->source_read += s.source_read;
+    out->sink_written += s.sink_written;
+    out->kernel_read += s.kernel_read;
+    out->kernel_written += s.kernel_written;
+}
+
+// The chunked tile grid (internal/exec/chunked.go:159-174).
+struct TileGrid {
+    int64_t W, H, tileW, tileH, tilesX, tiles;
+};
+
+TileGrid tile_grid(int64_t W, int64_t H, const strata_options *o)
+{
+    TileGrid g;
+    g.W = W;
+    g.H = H;
+    g.tileW = (o && o->tile_width > 0) ? std::min(o->tile_width, W) : W;
+    g.tileH = (o && o->tile_height > 0) ? std::min(o->tile_height, H) : H;
+    g.tilesX = cdiv(W, g.tileW);
+    g.tiles = g.tilesX * cdiv(H, g.tileH);
+    return g;
+}
+
+Rect tile_rect(const TileGrid &g, int64_t i)
+{
+    Rect t;
+    t.x0 = (i % g.tilesX) * g.tileW;
+    t.y0 = (i / g.tilesX) * g.tileH;
+    t.x1 = std::min(t.x0 + g.tileW, g.W);
+    t.y1 = std::min(t.y0 + g.tileH, g.H);
+    return t;
+}
+
+// One chunked tile's counters (chunked.go:335-374): the source window,
+// the sink tile, and the tile's own plan of bands.
+void tile_stats(strata_stats &s, int r, const TileGrid &g, int nin, int nout, const Rect &t)
+{
+    const int64_t rx0 = std::max<int64_t>(0, t.x0 - r), ry0 = std::max<int64_t>(0, t.y0 - r);
+    const int64_t rx1 = std::min<int64_t>(g.W, t.x1 + r), ry1 = std::min<int64_t>(g.H, t.y1 + r);
+    s.tiles += 1;
+    s.source_read += (rx1 - rx0) * (ry1 - ry0) * kBytesPerCell * nin;
+    const Plan p = new_plan(t.x1 - t.x0, t.y1 - t.y0, 0, 0);
+    for (int64_t b = 0; b < p.bands; ++b) {
+        Rect band = plan_band(p, b);
+        band.x0 += t.x0;
+        band.x1 += t.x0;
+        band.y0 += t.y0;
+        band.y1 += t.y0;
+        band_stats(s, r, g.W, g.H, nin, nout, band);
+    }
+    s.sink_written += (t.x1 - t.x0) * (t.y1 - t.y0) * kBytesPerCell * nout;
+}
+
+// ---------------------------------------------------------------------------
+// Device execution of one rectangle of the output
+// ---------------------------------------------------------------------------
+
+// Device views of one call's operands. The source's data[0] is raster cell
+// (sx, sy); output o's data[0] is raster cell (dx[o], dy[o]).
+struct Operands {
+    const strata_raster *src;
+    int64_t sx, sy;
+    const strata_raster *dst[2];
+    int64_t dx[2], dy[2];
+    int nout;
+    int64_t W, H;
+};
+
+int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st)
+{
+    Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
+    if (rg.w <= 0 || rg.h <= 0) return STRATA_OK;
+    const SrcView s = src_view(*op.src, op.sx, op.sy, op.W, op.H);
+    DstView d[2];
+    for (int o = 0; o < op.nout; ++o) {
+        d[o].p = op.dst[o]->data + (rc.y0 - op.dy[o]) * op.dst[o]->stride + (rc.x0 - op.dx[o]);
+        d[o].stride = op.dst[o]->stride;
+    }
+    HIP_TRY(launch_data(P, rg, s, d, op.nout, st), "launching the data kernel");
+    for (int o = 0; o < op.nout; ++o) {
+        if (!op.dst[o]->valid) continue;
+        const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o]);
+        const bool has_src = op.src->valid != nullptr;
+        const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy) : dm;
+        HIP_TRY(launch_validity(P.radius, rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");
+    }
+    return STRATA_OK;
+}
+
+bool cancelled(const strata_options *o) { return o && o->cancel && *o->cancel != 0; }
+
+// Device work to claim between cancellation checks: whole bands of the
+// plan, within one tile, at least this many cells.
+constexpr int64_t kUnitCells = 1 << 22;
+
+int check_operands(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
+                   const strata_options *opts)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
+    if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
+        return fail(STRATA_EINVAL, "engine: negative Options {TileWidth:%lld TileHeight:%lld Workers:%lld}",
+                    (long long)opts->tile_width, (long long)opts->tile_height, (long long)opts->workers);
+    for (int i = 0; i < ndst; ++i)
+        if ((rc = check_raster("dst", i, dst[i]))) return rc;
+    for (int i = 0; i < nsrc; ++i)
+        if ((rc = check_raster("src", i, src[i]))) return rc;
+    for (int i = 0; i < nsrc; ++i)
+        if (src[i].width != dst[0].width || src[i].height != dst[0].height)
+            return fail(STRATA_EINVAL, "engine: src[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)src[i].width, (long long)src[i].height, (long long)dst[0].width,
+                        (long long)dst[0].height);
+    for (int i = 1; i < ndst; ++i)
+        if (dst[i].width != dst[0].width || dst[i].height != dst[0].height)
+            return fail(STRATA_EINVAL, "engine: dst[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)dst[i].width, (long long)dst[i].height, (long long)dst[0].width,
+                        (long long)dst[0].height);
+    bool masked = false;
+    for (int i = 0; i < nsrc; ++i) masked = masked || src[i].valid;
+    for (int i = 0; i < ndst; ++i)
+        if (masked && !dst[i].valid)
+            return fail(STRATA_EINVAL,
+                        "engine: an input has a validity mask but dst[%d].Valid is nil; allocate it with "
+                        "raster.NewFloat32Like, or set Valid on its root raster before windowing",
+                        i);
+    return STRATA_OK;
+}
+
+int ensure_device()
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    return STRATA_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int strata_init(int device)
+{
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    HIP_TRY(hipFree(nullptr), "device init");
+    g_err.clear();
+    return STRATA_OK;
+}
+
+const char *strata_last_error(void) { return g_err.c_str(); }
+
+const char *strata_version(void) { return "strata-hip 0.1 gfx950"; }
+
+int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
+                      const strata_options *opts, int chunked, strata_stats *stats)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    if (width <= 0 || height <= 0) return fail(STRATA_EINVAL, "engine: sizes must be positive");
+    strata_stats s{};
+    if (chunked) {
+        const TileGrid g = tile_grid(width, height, opts);
+        for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, k->radius, g, nsrc, ndst, tile_rect(g, t));
+    } else {
+        const Plan p = new_plan(width, height, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
+        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, k->radius, width, height, nsrc, ndst, plan_band(p, b));
+    }
+    add_stats(stats, s, nsrc, ndst);
+    g_err.clear();
+    return STRATA_OK;
+}
+
+int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
+                   const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_operands(k, dst, ndst, src, nsrc, opts);
+    if (rc) return rc;
+    if ((rc = ensure_device())) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    const int64_t W = dst[0].width, H = dst[0].height;
+    const int r = k->radius;
+    const Params P = make_params(k);
+
+    // Host operands are mirrored on the device for the call.
+    bool any_host = src[0].mem == STRATA_MEM_HOST;
+    for (int o = 0; o < ndst; ++o) any_host = any_host || dst[o].mem == STRATA_MEM_HOST;
+    Staged ssrc, sdst[2];
+    const strata_raster *dsrc = &src[0];
+    const strata_raster *ddst[2] = {&dst[0], ndst > 1 ? &dst[1] : &dst[0]};
+    if (src[0].mem == STRATA_MEM_HOST) {
+        if ((rc = stage_in(src[0], ssrc, true, st))) return rc;
+        dsrc = &ssrc.dev;
+    }
+    for (int o = 0; o < ndst; ++o)
+        if (dst[o].mem == STRATA_MEM_HOST) {
+            if ((rc = stage_in(dst[o], sdst[o], false, st))) return rc;
+            ddst[o] = &sdst[o].dev;
+        }
+
+    Operands op{dsrc, 0, 0, {ddst[0], ddst[1]}, {0, 0}, {0, 0}, ndst, W, H};
+    strata_stats s{};
+    const Plan p = new_plan(W, H, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
+    const bool can_cancel = opts && opts->cancel;
+    int status = STRATA_OK;
+    if (!can_cancel) {
+        // The plan never changes a result, so the whole raster is one launch.
+        if ((rc = run_rect(P, op, Rect{0, 0, W, H}, st))) return rc;
+        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, r, W, H, nsrc, ndst, plan_band(p, b));
+    } else {
+        // Units of whole bands, in plan order, within one tile; the check
+        // comes before each claim and a claimed unit always completes.
+        int64_t b = 0;
+        while (b < p.bands) {
+            if (cancelled(opts)) {
+                status = STRATA_CANCELLED;
+                break;
+            }
+            Rect u = plan_band(p, b);
+            strata_stats us{};
+            band_stats(us, r, W, H, nsrc, ndst, u);
+            int64_t e = b + 1;
+            while (e < p.bands) {
+                const Rect nb = plan_band(p, e);
+                if (nb.x0 != u.x0 || nb.x1 != u.x1 || nb.y0 != u.y1) break;
+                if ((u.x1 - u.x0) * (u.y1 - u.y0) >= kUnitCells) break;
+                band_stats(us, r, W, H, nsrc, ndst, nb);
+                u.y1 = nb.y1;
+                ++e;
+            }
+            if ((rc = run_rect(P, op, u, st))) return rc;
+            HIP_TRY(hipStreamSynchronize(st), "waiting for a unit");
+            s.cells += us.cells;
+            s.bands += us.bands;
+            s.kernel_read += us.kernel_read;
+            s.kernel_written += us.kernel_written;
+            b = e;
+        }
+    }
+    for (int o = 0; o < ndst; ++o)
+        if (dst[o].mem == STRATA_MEM_HOST && (rc = stage_out(dst[o], sdst[o], st))) return rc;
+    if (any_host || !(opts && opts->async) || can_cancel) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+    add_stats(stats, s, nsrc, ndst);
+    g_err.clear();
+    return status;
+}
+
+int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int ndst, const strata_source *src,
+                           int nsrc, const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
+    if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
+        return fail(STRATA_EINVAL, "engine: negative Options");
+    const int64_t W = dst[0].width, H = dst[0].height;
+    if (W <= 0 || H <= 0)
+        return fail(STRATA_EINVAL, "engine: dst[0] is %lld×%lld; sizes must be positive", (long long)W, (long long)H);
+    for (int i = 0; i < nsrc; ++i) {
+        if (src[i].width != W || src[i].height != H)
+            return fail(STRATA_EINVAL, "engine: src[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)src[i].width, (long long)src[i].height, (long long)W, (long long)H);
+        if (!src[i].resident && !src[i].read) return fail(STRATA_EINVAL, "engine: src[%d] is a nil source", i);
+        if (src[i].resident) {
+            if ((rc = check_raster("src", i, *src[i].resident))) return rc;
+            if (src[i].resident->width != W || src[i].resident->height != H)
+                return fail(STRATA_EINVAL, "engine: src[%d]'s raster does not match its size", i);
+        }
+    }
+    bool masked = false;
+    for (int i = 0; i < nsrc; ++i) masked = masked || src[i].masked;
+    for (int i = 0; i < ndst; ++i) {
+        if (dst[i].width != W || dst[i].height != H)
+            return fail(STRATA_EINVAL, "engine: dst[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)dst[i].width, (long long)dst[i].height, (long long)W, (long long)H);
+        if (!dst[i].resident && !dst[i].write) return fail(STRATA_EINVAL, "engine: dst[%d] is a nil sink", i);
+        if (dst[i].resident && (rc = check_raster("dst", i, *dst[i].resident))) return rc;
+        if (masked && !dst[i].masked)
+            return fail(STRATA_EINVAL,
+                        "engine: a source is Masked but dst[%d] is not; its validity would be lost", i);
+    }
+    if ((rc = ensure_device())) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    const int r = k->radius;
+    const Params P = make_params(k);
+    const TileGrid g = tile_grid(W, H, opts);
+    const bool opts_cancel = opts && opts->cancel;
+
+    // Buffers for callback operands: inputs hold a tile grown by the
+    // radius, outputs a tile; a masked buffer's stride is a multiple of 64
+    // so every row's bits start a word (chunked.go:184-200).
+    const int64_t inW = std::min(W, g.tileW + 2 * r), inH = std::min(H, g.tileH + 2 * r);
+    auto round64 = [](int64_t n) { return (n + 63) & ~(int64_t)63; };
+    struct Buf {
+        int64_t stride = 0, cells = 0, words = 0;
+        float *hdata = nullptr;
+        uint64_t *hmask = nullptr;
+        float *ddata = nullptr;
+        uint64_t *dmask = nullptr;
+    };
+    std::vector<Buf> bufs(nsrc + ndst);
+    auto free_bufs = [&]() {
+        for (auto &b : bufs) {
+            if (b.hdata) (void)hipHostFree(b.hdata);
+            if (b.hmask) (void)hipHostFree(b.hmask);
+            if (b.ddata) (void)hipFree(b.ddata);
+            if (b.dmask) (void)hipFree(b.dmask);
+        }
+    };
+    struct Guard {
+        std::function<void()> f;
+        ~Guard() { f(); }
+    };
+    auto alloc_buf = [&](Buf &b, int64_t w, int64_t h, bool m) -> int {
+        b.stride = m ? round64(w) : w;
+        b.cells = (h - 1) * b.stride + w;
+        b.words = m ? cdiv(b.cells, 64) : 0;
+        HIP_TRY(hipHostMalloc((void **)&b.hdata, (size_t)b.cells * 4, hipHostMallocDefault), "pinned buffer");
+        HIP_TRY(hipMalloc((void **)&b.ddata, (size_t)b.cells * 4), "tile buffer");
+        if (m) {
+            HIP_TRY(hipHostMalloc((void **)&b.hmask, (size_t)b.words * 8, hipHostMallocDefault), "pinned mask");
+            HIP_TRY(hipMalloc((void **)&b.dmask, (size_t)b.words * 8), "tile mask");
+        }
+        return STRATA_OK;
+    };
+    int status = STRATA_OK;
+    {
+        Guard guard{free_bufs};
+        for (int i = 0; i < nsrc; ++i)
+            if (!src[i].resident && (rc = alloc_buf(bufs[i], inW, inH, src[i].masked))) return rc;
+        for (int o = 0; o < ndst; ++o)
+            if (!dst[o].resident && (rc = alloc_buf(bufs[nsrc + o], g.tileW, g.tileH, dst[o].masked))) return rc;
+
+        strata_stats s{};
+        for (int64_t t = 0; t < g.tiles; ++t) {
+            if (cancelled(opts)) {
+                status = STRATA_CANCELLED;
+                break;
+            }
+            const Rect tr = tile_rect(g, t);
+            const int64_t rx0 = std::max<int64_t>(0, tr.x0 - r), ry0 = std::max<int64_t>(0, tr.y0 - r);
+            const int64_t rx1 = std::min<int64_t>(W, tr.x1 + r), ry1 = std::min<int64_t>(H, tr.y1 + r);
+            const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
+
+            // The source window: resident (read in place) or read by the
+            // callback into the pinned buffer and copied to the device.
+            strata_raster sv{};
+            int64_t sx = 0, sy = 0;
+            if (src[0].resident) {
+                sv = *src[0].resident;
+                if (sv.mem == STRATA_MEM_HOST)
+                    return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+                if (!src[0].masked) sv.valid = nullptr;
+            } else {
+                Buf &b = bufs[0];
+                const int64_t ww = rx1 - rx0, hh = ry1 - ry0;
+                if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
+                if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
+                    status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
+                                  (long long)rx0, (long long)ry0);
+                    break;
+                }
+                const int64_t cells = (hh - 1) * b.stride + ww;
+                HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st), "tile in");
+                if (b.hmask)
+                    HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice, st),
+                            "tile mask in");
+                sv.data = b.ddata;
+                sv.width = ww;
+                sv.height = hh;
+                sv.stride = b.stride;
+                sv.valid = b.hmask ? b.dmask : nullptr;
+                sv.valid_offset = 0;
+                sx = rx0;
+                sy = ry0;
+            }
+            strata_raster dv[2]{};
+            Operands op{&sv, sx, sy, {&dv[0], &dv[ndst > 1 ? 1 : 0]}, {0, 0}, {0, 0}, ndst, W, H};
+            bool buffered = !src[0].resident;
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident) {
+                    dv[o] = *dst[o].resident;
+                    if (dv[o].mem == STRATA_MEM_HOST)
+                        return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                    if (!dst[o].masked) dv[o].valid = nullptr;
+                } else {
+                    Buf &b = bufs[nsrc + o];
+                    dv[o].data = b.ddata;
+                    dv[o].width = tw;
+                    dv[o].height = th;
+                    dv[o].stride = b.stride;
+                    dv[o].valid = b.dmask;
+                    dv[o].valid_offset = 0;
+                    op.dx[o] = tr.x0;
+                    op.dy[o] = tr.y0;
+                    buffered = true;
+                }
+            }
+            if ((rc = run_rect(P, op, tr, st))) return rc;
+            // Hand buffered outputs to their sinks.
+            bool waited = false;
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident) continue;
+                Buf &b = bufs[nsrc + o];
+                const int64_t cells = (th - 1) * b.stride + tw;
+                HIP_TRY(hipMemcpyAsync(b.hdata, b.ddata, (size_t)cells * 4, hipMemcpyDeviceToHost, st), "tile out");
+                if (b.hmask)
+                    HIP_TRY(hipMemcpyAsync(b.hmask, b.dmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyDeviceToHost, st),
+                            "tile mask out");
+            }
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident) continue;
+                if (!waited) {
+                    HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
+                    waited = true;
+                }
+                Buf &b = bufs[nsrc + o];
+                if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, tr.x0, tr.y0) != 0) {
+                    status = fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
+                                  (long long)tr.x0, (long long)tr.y0);
+                    break;
+                }
+            }
+            if (status != STRATA_OK) break;
+            // A callback source's pinned buffer is reused by the next tile,
+            // and a cancellable call observes its flag per finished tile.
+            if (!waited && (buffered || opts_cancel)) HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
+            tile_stats(s, r, g, nsrc, ndst, tr);
+        }
+        HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+        add_stats(stats, s, nsrc, ndst);
+    }
+    if (status == STRATA_OK) g_err.clear();
+    return status;
+}
+
+} // extern "C"

@@ -1,0 +1,614 @@
+"""engine: Options, Stats, contexts, sources and sinks, and the two entry
+points every operation funnels into — ProcessN (internal/exec/process.go:28)
+and ProcessChunked (internal/exec/chunked.go:33) — over the HIP library.
+
+Programming errors raise EnginePanic before anything is written, with the
+reference's messages (its panics). Returned values mirror Go's `error`:
+None on success, a Canceled or SourceError instance otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .raster import Float32Raster, RasterPanic, _is_torch, _ptr
+
+BYTES_PER_CELL = 4
+
+
+class EnginePanic(RasterPanic):
+    """The reference panics: a programming error, raised before any write."""
+
+
+class DeviceError(RuntimeError):
+    """The HIP runtime failed (a failure mode the CPU reference does not have)."""
+
+
+class Canceled(Exception):
+    """context.Canceled: the context was done before every unit was written."""
+
+    def __init__(self, msg: str = "context canceled"):
+        super().__init__(msg)
+
+
+class SourceError(Exception):
+    """A source or sink failed; wraps the operand and tile position like
+    internal/exec/chunked.go:379-393."""
+
+
+# -- context -----------------------------------------------------------------
+
+class Context:
+    """A cancellable context. The library polls its flag between units of
+    the plan (bands or tiles), never inside a kernel."""
+
+    def __init__(self):
+        self._flag = ctypes.c_int32(0)
+
+    def cancel(self) -> None:
+        self._flag.value = 1
+
+    def done(self) -> bool:
+        return self._flag.value != 0
+
+    def err(self) -> Optional[Exception]:
+        return Canceled() if self.done() else None
+
+    @property
+    def flag_address(self) -> int:
+        return ctypes.addressof(self._flag)
+
+
+def Background() -> Context:
+    return Context()
+
+
+def WithCancel(parent: Optional[Context] = None):
+    ctx = Context()
+    return ctx, ctx.cancel
+
+
+# -- Options and Stats ---------------------------------------------------------
+
+@dataclass
+class Stats:
+    """engine.Stats (engine/stats.go:77-104)."""
+    Cells: int = 0
+    Ideal: int = 0
+    Tiles: int = 0
+    Bands: int = 0
+    SourceRead: int = 0
+    SinkWritten: int = 0
+    KernelRead: int = 0
+    KernelWritten: int = 0
+
+    def Total(self) -> int:
+        return self.SourceRead + self.KernelRead + self.KernelWritten + self.SinkWritten
+
+    def Amplification(self) -> float:
+        return 0.0 if self.Ideal == 0 else self.Total() / self.Ideal
+
+    def BytesPerCell(self) -> float:
+        return 0.0 if self.Cells == 0 else self.Total() / self.Cells
+
+    def Halo(self, inputs: int) -> int:
+        if inputs <= 0:
+            return 0
+        return self.KernelRead - self.Cells * BYTES_PER_CELL * inputs
+
+    def Add(self, o: "Stats") -> None:
+        for f in ("Cells", "Ideal", "Tiles", "Bands", "SourceRead", "SinkWritten", "KernelRead",
+                  "KernelWritten"):
+            setattr(self, f, getattr(self, f) + getattr(o, f))
+
+    def _add_c(self, s: _lib.Stats) -> None:
+        self.Cells += s.cells
+        self.Ideal += s.ideal
+        self.Tiles += s.tiles
+        self.Bands += s.bands
+        self.SourceRead += s.source_read
+        self.SinkWritten += s.sink_written
+        self.KernelRead += s.kernel_read
+        self.KernelWritten += s.kernel_written
+
+
+@dataclass
+class Options:
+    """engine.Options (engine/engine.go:7-28). Workers is a scheduling hint:
+    the device decides how the plan's units run; results never change."""
+    TileWidth: int = 0
+    TileHeight: int = 0
+    Workers: int = 0
+    Stats: Optional[Stats] = None
+
+
+# -- kernel descriptors ------------------------------------------------------
+
+@dataclass
+class KernelDesc:
+    """A resolved kernel: what the reference's kernel constructors build
+    (terrain/*.go, focal/*.go, transfer/*.go), as the C ABI's strata_kernel."""
+    op: int
+    radius: int
+    flags: int = 0
+    k: Sequence[float] = ()
+    taps: Optional[np.ndarray] = None
+    nin: int = 1
+    nout: int = 1
+    name: str = ""
+
+    def to_c(self):
+        kc = _lib.Kernel()
+        kc.op, kc.radius, kc.flags = self.op, self.radius, self.flags
+        for i, v in enumerate(self.k):
+            kc.k[i] = float(np.float32(v))
+        taps = None
+        if self.taps is not None and len(self.taps):
+            taps = np.ascontiguousarray(self.taps, dtype=np.float32)
+            kc.ntaps = len(taps)
+            kc.taps = taps.ctypes.data
+        else:
+            kc.ntaps = 0
+            kc.taps = None
+        return kc, taps  # keep taps alive for the call
+
+
+# -- overlap checks (internal/overlap/overlap.go) ------------------------------
+
+def _span(r: Float32Raster) -> int:
+    return (r.Height - 1) * r.Stride + r.Width
+
+
+def _spans_meet(a0: int, an: int, b0: int, bn: int) -> bool:
+    return a0 < b0 + bn and b0 < a0 + an
+
+
+def data_spans(a: Float32Raster, b: Float32Raster) -> bool:
+    return _spans_meet(_ptr(a.Data), _span(a) * 4, _ptr(b.Data), _span(b) * 4)
+
+
+def _bit_addr(r: Float32Raster) -> int:
+    return _ptr(r.Valid) * 8 + r.ValidOffset
+
+
+def bit_spans(a: Float32Raster, b: Float32Raster) -> bool:
+    if a.Valid is None or b.Valid is None:
+        return False
+    return _spans_meet(_bit_addr(a), _span(a), _bit_addr(b), _span(b))
+
+
+DISJOINT, SAME, PARTIAL = 0, 1, 2
+
+
+def _relate(a0, a_stride, b0, b_stride, w, h) -> int:
+    if not _spans_meet(a0, (h - 1) * a_stride + w, b0, (h - 1) * b_stride + w):
+        return DISJOINT
+    if a0 == b0:
+        return SAME if (a_stride == b_stride or h == 1) else PARTIAL
+    if a_stride != b_stride:
+        return PARTIAL
+    stride = a_stride
+    diff = a0 - b0
+    dy, dx = int(diff / stride), diff - int(diff / stride) * stride  # Go truncating division
+    if dx < 0:
+        dy, dx = dy - 1, dx + stride
+
+    def rows_meet(d):
+        return -h < d < h
+
+    if (dx < w and rows_meet(dy)) or (dx > stride - w and rows_meet(dy + 1)):
+        return PARTIAL
+    return DISJOINT
+
+
+def data_relation(a: Float32Raster, b: Float32Raster) -> int:
+    a0, b0 = _ptr(a.Data), _ptr(b.Data)
+    if (a0 - b0) % 4 != 0:
+        return PARTIAL if _spans_meet(a0, _span(a) * 4, b0, _span(b) * 4) else DISJOINT
+    return _relate(a0 // 4, a.Stride, b0 // 4, b.Stride, a.Width, a.Height)
+
+
+def bits_relation(a: Float32Raster, b: Float32Raster) -> int:
+    if a.Valid is None or b.Valid is None:
+        return DISJOINT
+    return _relate(_bit_addr(a), a.Stride, _bit_addr(b), b.Stride, a.Width, a.Height)
+
+
+# -- checks (internal/exec/process.go:37-135) --------------------------------------
+
+def _check_kernel(ndst: int, nsrc: int, k: KernelDesc, opts: Options) -> None:
+    if k is None:
+        raise EnginePanic("engine: nil kernel")
+    if k.radius < 0:
+        raise EnginePanic(f"engine: kernel radius {k.radius} is negative")
+    if nsrc != k.nin or ndst != k.nout:
+        raise EnginePanic(f"engine: kernel takes {k.nin} inputs and {k.nout} outputs, got {nsrc} and {ndst}")
+    if opts.TileWidth < 0 or opts.TileHeight < 0 or opts.Workers < 0:
+        raise EnginePanic(f"engine: negative Options {opts}")
+
+
+def _require_raster(name: str, i: int, r: Float32Raster, ref: Float32Raster, ref_name: str) -> None:
+    err = r.Validate()
+    if err is not None:
+        raise EnginePanic(f"engine: {name}[{i}]: {err}")
+    if r.Width != ref.Width or r.Height != ref.Height:
+        raise EnginePanic(f"engine: {name}[{i}] is {r.Width}×{r.Height}, {ref_name} is {ref.Width}×{ref.Height}")
+
+
+def check(dst: List[Float32Raster], src: List[Float32Raster], k: KernelDesc, opts: Options) -> None:
+    _check_kernel(len(dst), len(src), k, opts)
+    r = k.radius
+    for i, d in enumerate(dst):
+        _require_raster("dst", i, d, dst[0], "dst[0]")
+    masked = False
+    for i, s in enumerate(src):
+        _require_raster("src", i, s, dst[0], "dst[0]")
+        masked = masked or s.Valid is not None
+    for i, d in enumerate(dst):
+        if masked and d.Valid is None:
+            raise EnginePanic(
+                f"engine: an input has a validity mask but dst[{i}].Valid is nil; allocate it with "
+                "raster.NewFloat32Like, or set Valid on its root raster before windowing")
+    devs = {bool(x.on_device) for x in list(dst) + list(src)}
+    if len(devs) > 1:
+        raise EnginePanic("engine: operands must all be in device memory or all in host memory")
+    for i, d in enumerate(dst):
+        for j in range(i + 1, len(dst)):
+            meet = data_spans(d, dst[j]) if r > 0 else data_relation(d, dst[j]) != DISJOINT
+            if meet:
+                raise EnginePanic(f"engine: dst[{i}] and dst[{j}] share Data")
+            meet = bit_spans(d, dst[j]) if r > 0 else bits_relation(d, dst[j]) != DISJOINT
+            if meet:
+                raise EnginePanic(f"engine: dst[{i}] and dst[{j}] share validity bits")
+        for j, s in enumerate(src):
+            if r > 0:
+                if data_spans(d, s):
+                    raise EnginePanic(f"engine: dst[{i}] and src[{j}] share Data; a kernel with radius {r} "
+                                      "would read neighbours it has already overwritten")
+                if bit_spans(d, s):
+                    raise EnginePanic(f"engine: dst[{i}] and src[{j}] share validity bits")
+                continue
+            rel = data_relation(d, s)
+            if rel == PARTIAL:
+                raise EnginePanic(f"engine: dst[{i}] overlaps src[{j}] at a different offset or stride")
+            if rel == SAME and k.nout > 1:
+                raise EnginePanic(f"engine: dst[{i}] is src[{j}]; only a kernel with one output runs in place")
+            if bits_relation(d, s) == PARTIAL:
+                raise EnginePanic(f"engine: dst[{i}] validity bits overlap src[{j}]'s at a different offset "
+                                  "or stride")
+
+
+# -- the calls ----------------------------------------------------------------
+
+def _stream_handle(rasters) -> int:
+    for r in rasters:
+        if r is not None and r.on_device:
+            import torch
+            return torch.cuda.current_stream(r.Data.device).cuda_stream
+    return 0
+
+
+def _c_options(opts: Options, ctx: Optional[Context], stream: int, async_: bool = False) -> _lib.Options:
+    o = _lib.Options()
+    o.tile_width, o.tile_height, o.workers = opts.TileWidth, opts.TileHeight, opts.Workers
+    o.stream = stream or None
+    o.cancel = ctx.flag_address if ctx is not None else None
+    o.async_ = 1 if async_ else 0
+    return o
+
+
+def _raise_status(rc: int) -> None:
+    msg = _lib.last_error()
+    if rc == _lib.EINVAL:
+        raise EnginePanic(msg)
+    raise DeviceError(msg or f"strata: status {rc}")
+
+
+def ProcessN(ctx: Optional[Context], dst: List[Float32Raster], src: List[Float32Raster], k: KernelDesc,
+             opts: Optional[Options] = None, *, async_: bool = False) -> Optional[Exception]:
+    """exec.ProcessN: run k over every cell of dst from src."""
+    opts = opts or Options()
+    check(dst, src, k, opts)
+    L = _lib.lib()
+    kc, taps = k.to_c()
+    cd = (_lib.Raster * len(dst))(*[d.to_c() for d in dst])
+    cs = (_lib.Raster * len(src))(*[s.to_c() for s in src])
+    st = _lib.Stats()
+    co = _c_options(opts, ctx if (ctx is not None and not async_) else None, _stream_handle(dst + src), async_)
+    rc = L.strata_process(ctypes.byref(kc), cd, len(dst), cs, len(src), ctypes.byref(co), ctypes.byref(st))
+    del taps
+    if rc < 0:
+        _raise_status(rc)
+    if opts.Stats is not None:
+        opts.Stats._add_c(st)
+    if rc == _lib.CANCELLED:
+        return Canceled()
+    return None
+
+
+def PlanStats(k: KernelDesc, width: int, height: int, opts: Optional[Options] = None,
+              chunked: bool = False) -> Stats:
+    """The Stats a completed call adds, in closed form."""
+    opts = opts or Options()
+    kc, taps = k.to_c()
+    co = _c_options(opts, None, 0)
+    st = _lib.Stats()
+    rc = _lib.lib().strata_plan_stats(ctypes.byref(kc), k.nout, k.nin, width, height, ctypes.byref(co),
+                                      1 if chunked else 0, ctypes.byref(st))
+    if rc < 0:
+        _raise_status(rc)
+    out = Stats()
+    out._add_c(st)
+    return out
+
+
+# -- sources and sinks (engine/source.go) -----------------------------------------
+
+class RasterSource:
+    """engine.RasterSource: Size, Masked and ReadWindow(ctx, dst, x, y)."""
+
+    def Size(self):
+        raise NotImplementedError
+
+    def Masked(self) -> bool:
+        raise NotImplementedError
+
+    def ReadWindow(self, ctx, dst: Float32Raster, x: int, y: int) -> Optional[Exception]:
+        raise NotImplementedError
+
+
+class RasterSink:
+    def Size(self):
+        raise NotImplementedError
+
+    def Masked(self) -> bool:
+        raise NotImplementedError
+
+    def WriteWindow(self, ctx, src: Float32Raster, x: int, y: int) -> Optional[Exception]:
+        raise NotImplementedError
+
+
+def _copy_cells(dst: Float32Raster, src: Float32Raster, valid: bool) -> None:
+    """Host copy of cells (and bits) between equal-size rasters (engine/source.go:152-183)."""
+    from .raster import MaskSet, MaskGet
+    for y in range(dst.Height):
+        dst.Data[y * dst.Stride:y * dst.Stride + dst.Width] = src.Data[y * src.Stride:y * src.Stride + src.Width]
+    if valid and dst.Valid is not None:
+        _copy_validity(dst, src)
+
+
+def _copy_validity(dst: Float32Raster, src: Float32Raster) -> None:
+    from .raster import unpack_bits
+    dv = dst.Valid
+    for y in range(dst.Height):
+        if src.Valid is None:
+            bits = np.ones(dst.Width, dtype=bool)
+        else:
+            bits = unpack_bits(src.Valid, src.ValidOffset + y * src.Stride, src.Width)
+        _put_bits(dv, dst.ValidOffset + y * dst.Stride, bits)
+
+
+def _put_bits(m: np.ndarray, off: int, bits: np.ndarray) -> None:
+    """Write a run of bits into a host uint64 mask, leaving other bits alone."""
+    n = len(bits)
+    if n == 0:
+        return
+    w0, w1 = off >> 6, (off + n - 1) >> 6
+    words = m[w0:w1 + 1].copy()
+    allb = np.unpackbits(words.view(np.uint8), bitorder="little").astype(bool)
+    s = off - (w0 << 6)
+    allb[s:s + n] = bits
+    m[w0:w1 + 1] = np.packbits(allb.reshape(-1, 8), axis=1, bitorder="little").reshape(-1).view(np.uint64)
+
+
+class MemorySource(RasterSource):
+    """A source over a raster in memory (engine/source.go:76-107). A raster
+    in device memory is read in place by the device tile queue."""
+
+    def __init__(self, r: Float32Raster):
+        err = r.Validate()
+        if err is not None:
+            raise EnginePanic(f"engine: NewMemorySource: {err}")
+        self.r = r
+
+    def Raster(self) -> Float32Raster:
+        return self.r
+
+    def Size(self):
+        return self.r.Width, self.r.Height
+
+    def Masked(self) -> bool:
+        return self.r.Valid is not None
+
+    def ReadWindow(self, ctx, dst: Float32Raster, x: int, y: int):
+        win = self.r.Window(x, y, dst.Width, dst.Height)
+        _copy_cells(dst, win if not win.on_device else win.to("cpu"), dst.Valid is not None)
+        return None
+
+
+class MemorySink(RasterSink):
+    def __init__(self, r: Float32Raster):
+        err = r.Validate()
+        if err is not None:
+            raise EnginePanic(f"engine: NewMemorySink: {err}")
+        self.r = r
+        self._mu = threading.Lock()
+
+    def Raster(self) -> Float32Raster:
+        return self.r
+
+    def Size(self):
+        return self.r.Width, self.r.Height
+
+    def Masked(self) -> bool:
+        return self.r.Valid is not None
+
+    def WriteWindow(self, ctx, src: Float32Raster, x: int, y: int):
+        dst = self.r.Window(x, y, src.Width, src.Height)
+        _copy_cells(dst, src, False)
+        if dst.Valid is not None:
+            with self._mu:
+                _copy_validity(dst, src)
+        return None
+
+
+NewMemorySource = MemorySource
+NewMemorySink = MemorySink
+
+
+def _resident(x) -> Optional[Float32Raster]:
+    if isinstance(x, (MemorySource, MemorySink)) and x.r.on_device:
+        return x.r
+    return None
+
+
+def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[RasterSource], k: KernelDesc,
+                   opts: Optional[Options] = None) -> Optional[Exception]:
+    """exec.ProcessChunked: tiles read with their halo, computed on the
+    device, written to the sinks; a memory source or sink in device memory
+    is read or written in place (the device tile queue)."""
+    opts = opts or Options()
+    _check_kernel(len(dst), len(src), k, opts)
+    for i, d in enumerate(dst):
+        if d is None:
+            raise EnginePanic(f"engine: dst[{i}] is a nil sink")
+    for i, s in enumerate(src):
+        if s is None:
+            raise EnginePanic(f"engine: src[{i}] is a nil source")
+    w, h = dst[0].Size()
+    if w <= 0 or h <= 0:
+        raise EnginePanic(f"engine: dst[0] is {w}×{h}; sizes must be positive")
+    masked = False
+    for i, s in enumerate(src):
+        sw, sh = s.Size()
+        if (sw, sh) != (w, h):
+            raise EnginePanic(f"engine: src[{i}] is {sw}×{sh}, dst[0] is {w}×{h}")
+        masked = masked or s.Masked()
+    for i, d in enumerate(dst):
+        sw, sh = d.Size()
+        if (sw, sh) != (w, h):
+            raise EnginePanic(f"engine: dst[{i}] is {sw}×{sh}, dst[0] is {w}×{h}")
+        if masked and not d.Masked():
+            raise EnginePanic(f"engine: a source is Masked but dst[{i}] is not; its validity would be lost")
+    for i, d in enumerate(dst):
+        if not isinstance(d, MemorySink):
+            continue
+        a = d.r
+        for j in range(i + 1, len(dst)):
+            o = dst[j]
+            if isinstance(o, MemorySink) and (data_spans(a, o.r) or _words_meet(a, o.r)):
+                raise EnginePanic(f"engine: memory sinks dst[{i}] and dst[{j}] share memory")
+        for j, s in enumerate(src):
+            if isinstance(s, MemorySource) and (data_spans(a, s.r) or _words_meet(a, s.r)):
+                raise EnginePanic(
+                    f"engine: memory sink dst[{i}] shares memory with memory source src[{j}]; tiles would read "
+                    "cells other tiles write (use a Tiled entry point on the rasters instead)")
+
+    keep = []  # objects that must outlive the call
+    errors: List[Exception] = []
+
+    def make_read(s: RasterSource, j: int):
+        def read(user, buf, ww, hh, stride, valid, voff, x, y):
+            try:
+                n = (hh - 1) * stride + ww
+                data = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_float)), shape=(n,))
+                vm = None
+                if valid:
+                    vm = np.ctypeslib.as_array(ctypes.cast(valid, ctypes.POINTER(ctypes.c_uint64)),
+                                               shape=((voff + n + 63) >> 6,))
+                v = Float32Raster(data, ww, hh, stride, vm, voff)
+                err = s.ReadWindow(ctx, v, x, y)
+                if err is not None:
+                    errors.append(SourceError(f"engine: reading src[{j}] at ({x}, {y}): {err}"))
+                    return 1
+                if vm is not None and not s.Masked():
+                    _copy_validity(v, Float32Raster(data, ww, hh, stride, None, 0))
+                return 0
+            except Exception as e:  # a panicking source
+                errors.append(e)
+                return 1
+        return _lib.READ_FN(read)
+
+    def make_write(d: RasterSink, j: int):
+        def write(user, buf, ww, hh, stride, valid, voff, x, y):
+            try:
+                n = (hh - 1) * stride + ww
+                data = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_float)), shape=(n,))
+                vm = None
+                if valid:
+                    vm = np.ctypeslib.as_array(ctypes.cast(valid, ctypes.POINTER(ctypes.c_uint64)),
+                                               shape=((voff + n + 63) >> 6,))
+                err = d.WriteWindow(ctx, Float32Raster(data, ww, hh, stride, vm, voff), x, y)
+                if err is not None:
+                    errors.append(SourceError(f"engine: writing dst[{j}] at ({x}, {y}): {err}"))
+                    return 1
+                return 0
+            except Exception as e:
+                errors.append(e)
+                return 1
+        return _lib.WRITE_FN(write)
+
+    csrc = (_lib.Source * len(src))()
+    for j, s in enumerate(src):
+        sw, sh = s.Size()
+        c = csrc[j]
+        c.width, c.height, c.masked = sw, sh, 1 if s.Masked() else 0
+        res = _resident(s)
+        if res is not None:
+            rc_ = res.to_c()
+            keep.append(rc_)
+            c.resident = ctypes.pointer(rc_)
+        else:
+            fn = make_read(s, j)
+            keep.append(fn)
+            c.read = fn
+    cdst = (_lib.Sink * len(dst))()
+    for j, d in enumerate(dst):
+        sw, sh = d.Size()
+        c = cdst[j]
+        c.width, c.height, c.masked = sw, sh, 1 if d.Masked() else 0
+        res = _resident(d)
+        if res is not None:
+            rc_ = res.to_c()
+            keep.append(rc_)
+            c.resident = ctypes.pointer(rc_)
+        else:
+            fn = make_write(d, j)
+            keep.append(fn)
+            c.write = fn
+    kc, taps = k.to_c()
+    rasters = [x.r for x in list(src) + list(dst) if isinstance(x, (MemorySource, MemorySink))]
+    co = _c_options(opts, ctx, _stream_handle(rasters))
+    st = _lib.Stats()
+    rc = _lib.lib().strata_process_chunked(ctypes.byref(kc), cdst, len(dst), csrc, len(src), ctypes.byref(co),
+                                           ctypes.byref(st))
+    del keep, taps
+    if opts.Stats is not None:
+        opts.Stats._add_c(st)
+    if errors and not isinstance(errors[0], SourceError):
+        raise errors[0]
+    if rc == _lib.EIO:
+        return errors[0] if errors else SourceError(_lib.last_error())
+    if rc < 0:
+        _raise_status(rc)
+    if rc == _lib.CANCELLED:
+        return Canceled()
+    return None
+
+
+def _words_meet(a: Float32Raster, b: Float32Raster) -> bool:
+    if a.Valid is None or b.Valid is None:
+        return False
+
+    def words(r):
+        first = _bit_addr(r) >> 6
+        return first, ((_bit_addr(r) + _span(r) + 63) >> 6) - first
+
+    a0, an = words(a)
+    b0, bn = words(b)
+    return _spans_meet(a0, an, b0, bn)
