@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the raster stencil path on one MI355X.
+
+Workload (the headline): terrain.SlopeChunked over a device-resident
+20000×20000 float32 DEM — seeded diamond-square fBm (Hurst 0.8) scaled to
+0..3000 m, 0.5% NoData in circular blobs of radius 5..50 — Horn 3×3,
+CellSize 30, degrees, engine.Options{TileHeight: 256}. A step is one Slope
+over the whole raster (data and validity kernels). `value` is Gcells/s
+over the timed steps, inputs already resident in HBM.
+
+Also reported: the roofline of the dominant kernel (the Slope data
+kernel) from HIP events the library records around each of its launches
+on the launch stream, the rest of the terrain suite (Aspect, Hillshade,
+profile/plan/mean Curvature, Gradient) on the same DEM, and the CPU
+baseline: the oracle (a scalar C restatement of the reference's kernels)
+timed on the host cores over a bounded sample of the same rows, whose
+output is also compared bit for bit with the GPU's.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcells/s + % HBM peak (Slope 20000²); p50/p99 tile latency at fixed QPS"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=20000, help="DEM width and height")
+    ap.add_argument("--tile-height", type=int, default=256)
+    ap.add_argument("--suite", choices=["terrain", "slope"], default="terrain",
+                    help="terrain: Slope then the rest of the suite; slope: Slope only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-rows", type=int, default=2000, help="rows in the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work (thread-seconds) to aim for")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device copy of the DEM (for HBM counter calibration)")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported. "
+              "The path shards by row strips with a radius-r halo exchange; see DESIGN.md.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from strata_amd import _lib, synth, terrain
+    from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked, Stats
+    from strata_amd.raster import Float32Raster
+
+    dev = torch.device("cuda:0")
+    L = _lib.lib()
+    assert L.strata_init(0) == 0, _lib.last_error()
+    N = args.size
+    cells = N * N
+
+    t0 = time.time()
+    dem_data = synth.fbm_dem(N, N, hurst=0.8, seed=1, lo=0.0, hi=3000.0, device=dev)
+    valid, covered = synth.blob_nodata(N, N, frac=0.005, rmin=5, rmax=50, seed=2, device=dev)
+    torch.cuda.synchronize()
+    log(f"bench: DEM {N}x{N} generated in {time.time() - t0:.1f}s, NoData {covered * 100:.3f}%")
+
+    dem = Float32Raster(dem_data, N, N, N, valid, 0)
+    out = Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                        torch.zeros_like(valid), 0)
+    src, sink = MemorySource(dem), MemorySink(out)
+    eopts = Options(TileHeight=args.tile_height)
+
+    def run_op(k, sinks):
+        err = ProcessChunked(None, sinks, [src], k, eopts, async_=True)
+        assert err is None
+
+    def time_op(k, sinks, steps, warmup):
+        for _ in range(warmup):
+            run_op(k, sinks)
+        torch.cuda.synchronize()
+        timing = _lib.KernelTiming().begin()
+        t = time.perf_counter()
+        for _ in range(steps):
+            run_op(k, sinks)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        timing.end()
+        return el, timing
+
+    slope = terrain.slopeOp(terrain.SlopeOptions(CellSize=30, Units=terrain.SlopeDegrees))
+    el, timing = time_op(slope, [sink], args.steps, args.warmup)
+    ms_step = el * 1e3 / args.steps
+    value = cells * args.steps / el / 1e9
+    kern_ms = timing.data_ms / max(1, timing.data_launches)
+    vkern_ms = timing.validity_ms / max(1, timing.validity_launches)
+    data_bytes = 8 * cells  # 4 B read + 4 B written per cell
+    achieved = data_bytes / (kern_ms * 1e-3) / 1e9
+    op_bytes = 8.25 * cells  # plus 1/8 B mask in and out
+    log(f"bench: Slope {ms_step:.3f} ms/step, {value:.1f} Gcells/s; data kernel {kern_ms:.3f} ms "
+        f"({achieved:.0f} GB/s), validity kernel {vkern_ms:.3f} ms")
+
+    # Statistics of the configuration, in the reference's closed form.
+    st = Stats()
+    run_op_stats = ProcessChunked(None, [sink], [src], slope, Options(TileHeight=args.tile_height, Stats=st))
+    assert run_op_stats is None
+
+    # The CPU baseline, and the GPU Slope output checked against it, before
+    # the rest of the suite reuses the output raster.
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(args, dem, out, slope, N, log)
+
+    suite = {}
+    if args.suite == "terrain":
+        out2 = Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                             torch.zeros_like(valid), 0)
+        ops = [
+            ("Aspect", terrain.aspectOp(terrain.AspectOptions(CellSize=30)), [sink]),
+            ("Hillshade", terrain.hillshadeOp(terrain.HillshadeOptions(CellSize=30, Azimuth=315, Altitude=45,
+                                                                       ZFactor=1)), [sink]),
+            ("CurvatureProfile", terrain.curvatureOp(terrain.CurvatureOptions(
+                CellSize=30, Type=terrain.CurvatureProfile)), [sink]),
+            ("CurvaturePlan", terrain.curvatureOp(terrain.CurvatureOptions(
+                CellSize=30, Type=terrain.CurvaturePlan)), [sink]),
+            ("CurvatureMean", terrain.curvatureOp(terrain.CurvatureOptions(
+                CellSize=30, Type=terrain.CurvatureMean)), [sink]),
+            ("Gradient", terrain.gradientOp(terrain.GradientOptions(CellSize=30)), [sink, MemorySink(out2)]),
+        ]
+        suite["Slope"] = {"gcells_s": round(value, 2), "op_hbm_frac": round(op_bytes / (el / args.steps) / 1e9 /
+                                                                              HBM_PEAK_GBS, 4),
+                          "data_kernel_ms": round(kern_ms, 4), "validity_kernel_ms": round(vkern_ms, 4)}
+        n_steps = max(5, args.steps // 2)
+        for name, k, sinks in ops:
+            e2, t2 = time_op(k, sinks, n_steps, 2)
+            nout = len(sinks)
+            b = (4 + 4 * nout + 0.125 * (1 + nout)) * cells
+            suite[name] = {"gcells_s": round(cells * n_steps / e2 / 1e9, 2),
+                           "op_hbm_frac": round(b / (e2 / n_steps) / 1e9 / HBM_PEAK_GBS, 4),
+                           "data_kernel_ms": round(t2.data_ms / max(1, t2.data_launches), 4)}
+            log(f"bench: {name} {suite[name]}")
+        del out2
+
+    if args.calibrate_copy:
+        tmp = torch.empty_like(dem_data)
+        tmp.copy_(dem_data)
+        torch.cuda.synchronize()
+        del tmp
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": (f"synthetic: seeded diamond-square fBm {N}x{N} (Hurst 0.8, 0-3000 m), "
+                 f"{covered * 100:.2f}% NoData in circular blobs of radius 5-50"),
+        "config": {"workload": f"terrain.SlopeChunked {N}x{N} float32 DEM, device-resident, "
+                               f"TileHeight {args.tile_height}, Horn 3x3, CellSize 30, degrees",
+                   "width": N, "height": N, "tile_height": args.tile_height,
+                   "stats": {"cells": st.Cells, "tiles": st.Tiles, "bands": st.Bands,
+                             "amplification": round(st.Amplification(), 4)}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "horn_sweep<SLOPE> (data kernel)", "bytes_per_launch": data_bytes,
+                     "kernel_ms": round(kern_ms, 4), "launches": timing.data_launches},
+        "op_roofline": {"bytes_per_step": op_bytes, "frac": round(op_bytes / (el / args.steps) / 1e9 /
+                                                                   HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+        "parity_sample": parity,
+        "suite": suite,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(args, dem, out, slope, N, log):
+    """The oracle timed on the host cores over the first rows of the same
+    DEM; its output is compared with the GPU's on the rows both compute
+    as interior."""
+    import numpy as np
+
+    from oracle import oracle
+    from strata_amd.raster import Float32Raster
+
+    rows = min(args.cpu_rows, N)
+    n = rows * N
+    words = (n + 63) // 64
+    src = Float32Raster(dem.Data[:n].cpu().numpy().copy(), N, rows, N,
+                        dem.Valid[:words].cpu().numpy().view(np.uint64).copy(), 0)
+    dst = Float32Raster(np.zeros(n, np.float32), N, rows, N, np.zeros(words, np.uint64), 0)
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    t = time.perf_counter()
+    oracle.process(slope, [dst], [src], threads=threads)
+    one = time.perf_counter() - t
+    reps = max(1, min(int(args.cpu_seconds / max(one * threads, 1e-6)), 200))
+    t = time.perf_counter()
+    for _ in range(reps):
+        oracle.process(slope, [dst], [src], threads=threads)
+    el = time.perf_counter() - t
+    rate = n * reps / el / 1e9
+    log(f"bench: CPU oracle {threads} threads, {rows}x{N} sample x{reps}: {rate:.3f} Gcells/s")
+
+    # Rows 1..rows-2 are interior in both the sample and the full raster.
+    got = out.Data[N:(rows - 1) * N].cpu().numpy()
+    want = dst.Data[N:(rows - 1) * N]
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    from strata_amd.raster import unpack_bits
+    gv = unpack_bits(out.Valid, N, (rows - 2) * N)
+    wv = unpack_bits(dst.Valid, N, (rows - 2) * N)
+    parity = {"rows": rows - 2, "cells": int(same.size), "data_equal": bool(same.all()),
+              "valid_equal": bool(np.array_equal(gv, wv))}
+    cpu = {"value": round(rate, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+           "sample": f"oracle (scalar C restatement) Slope on rows 0-{rows - 1} of the same DEM ({rows}x{N}), "
+                     f"{reps} passes, {threads} threads"}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    sys.exit(main())

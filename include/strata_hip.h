@@ -225,6 +225,29 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
 int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
                       const strata_options *opts, int chunked, strata_stats *stats);
 
+/* ------------------------------------------------------------------ */
+/* Self-checks (no counterpart in the reference)                       */
+/* ------------------------------------------------------------------ */
+
+/* Compare a device fast path with the full correctly rounded sequence it
+ * stands in for: kind 0 the square root over the float bit patterns
+ * [lo, lo+n), kind 1 the small-argument arctangent over [lo, lo+n),
+ * kind 2 the quotient over n hashed operand pairs (seed lo). Reports the
+ * number of differing results, the first differing index and how many
+ * inputs were in the fast path's range. */
+int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, uint64_t *first, uint64_t *checked);
+
+/* ------------------------------------------------------------------ */
+/* Measurement (no counterpart in the reference, which has only Stats)  */
+/* ------------------------------------------------------------------ */
+
+/* Start recording HIP events around every data and validity kernel launch
+ * (on the launch's own stream), discarding earlier records. */
+int strata_timing_begin(void);
+/* Stop recording; wait for the recorded launches and return the summed
+ * kernel durations and launch counts. */
+int strata_timing_end(double *data_ms, int64_t *data_launches, double *validity_ms, int64_t *validity_launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -379,11 +379,12 @@ static float lookup_cell(float v, const float *xs, const float *ys, int n)
  * cleared. Only the outputs' own cells and bits are written.
  * ------------------------------------------------------------------- */
 static inline int get_bit(const uint64_t *m, int64_t i) { return (int)((m[i >> 6] >> (i & 63)) & 1u); }
+/* Atomic, so threads owning neighbouring rows may share a word. */
 static inline void put_bit(uint64_t *m, int64_t i, int v)
 {
     uint64_t b = (uint64_t)1 << (i & 63);
-    if (v) m[i >> 6] |= b;
-    else m[i >> 6] &= ~b;
+    if (v) __atomic_fetch_or(&m[i >> 6], b, __ATOMIC_RELAXED);
+    else __atomic_fetch_and(&m[i >> 6], ~b, __ATOMIC_RELAXED);
 }
 
 static int n_outputs(const orc_kernel *k) { return k->op == ORC_OP_GRADIENT ? 2 : 1; }
@@ -413,8 +414,7 @@ typedef struct {
     int64_t y0, y1;
 } rows_job;
 
-/* Data only: validity bits are written by validity_rows, serially,
- * since neighbouring rows can share mask words. */
+/* Data of the rows; validity_rows writes their bits. */
 static void data_rows(const rows_job *j)
 {
     const orc_kernel *k = j->k;
@@ -477,6 +477,7 @@ static void validity_rows(const rows_job *j)
 static void *data_thread(void *arg)
 {
     data_rows((const rows_job *)arg);
+    validity_rows((const rows_job *)arg);
     return NULL;
 }
 
@@ -495,7 +496,7 @@ int orc_process_rows(const orc_kernel *k, orc_raster *dst, int ndst, const orc_r
     if (nthreads < 1) nthreads = 1;
     if (nthreads == 1) {
         rows_job j = {k, dst, ndst, src, y0, y1};
-        data_rows(&j);
+        data_thread(&j);
     } else {
         pthread_t th[256];
         rows_job jobs[256];
@@ -507,15 +508,13 @@ int orc_process_rows(const orc_kernel *k, orc_raster *dst, int ndst, const orc_r
             if (b <= a) continue;
             jobs[used] = (rows_job){k, dst, ndst, src, a, b};
             if (pthread_create(&th[used], NULL, data_thread, &jobs[used]) != 0) {
-                data_rows(&jobs[used]);
+                data_thread(&jobs[used]);
                 continue;
             }
             used++;
         }
         for (int t = 0; t < used; t++) pthread_join(th[t], NULL);
     }
-    rows_job j = {k, dst, ndst, src, y0, y1};
-    validity_rows(&j);
     return 0;
 }
 

@@ -134,6 +134,9 @@ EXPORTS = (
     "strata_process",
     "strata_process_chunked",
     "strata_plan_stats",
+    "strata_selfcheck",
+    "strata_timing_begin",
+    "strata_timing_end",
 )
 
 _lib = None
@@ -164,6 +167,14 @@ def lib() -> ctypes.CDLL:
     L.strata_plan_stats.argtypes = [POINTER(Kernel), c_int, c_int, c_int64, c_int64, POINTER(Options), c_int,
                                     POINTER(Stats)]
     L.strata_plan_stats.restype = c_int
+    L.strata_selfcheck.argtypes = [c_int, ctypes.c_uint64, ctypes.c_uint64, POINTER(ctypes.c_uint64),
+                                   POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64)]
+    L.strata_selfcheck.restype = c_int
+    L.strata_timing_begin.argtypes = []
+    L.strata_timing_begin.restype = c_int
+    L.strata_timing_end.argtypes = [POINTER(ctypes.c_double), POINTER(c_int64), POINTER(ctypes.c_double),
+                                    POINTER(c_int64)]
+    L.strata_timing_end.restype = c_int
     _bind_optional(L)
     _lib = L
     return L
@@ -179,6 +190,25 @@ def _bind_optional(L) -> None:
 
 
 _OPTIONAL: list = []
+
+
+class KernelTiming:
+    """Sums of device kernel durations recorded by the library between
+    begin() and end(): HIP events around each launch on its own stream."""
+
+    def begin(self) -> "KernelTiming":
+        lib().strata_timing_begin()
+        return self
+
+    def end(self):
+        dms, vms = ctypes.c_double(), ctypes.c_double()
+        dn, vn = c_int64(), c_int64()
+        rc = lib().strata_timing_end(ctypes.byref(dms), ctypes.byref(dn), ctypes.byref(vms), ctypes.byref(vn))
+        if rc != OK:
+            raise RuntimeError(last_error())
+        self.data_ms, self.data_launches = dms.value, dn.value
+        self.validity_ms, self.validity_launches = vms.value, vn.value
+        return self
 
 
 def last_error() -> str:

@@ -56,33 +56,250 @@ __device__ __forceinline__ float from_right(float v, float edge)
 // ---------------------------------------------------------------------------
 // horn_sweep: radius-1 terrain kernels
 // ---------------------------------------------------------------------------
+//
+// Layout: a block of 256 lanes covers 512 columns, two adjacent columns per
+// lane, and walks down a strip of kSweepRows rows. Each input row is loaded
+// once per strip (a dwordx2 per lane when rows are 8-byte aligned); the cell
+// left of a lane's pair comes from lane-1 and the cell right of it from
+// lane+1 through DPP wave shifts, and each wave's two outer columns from one
+// extra load by lanes 0 and 63. Rows live in a ring of four register sets,
+// unrolled so no row ever moves between registers, with the next row's
+// loads in flight while the current one is computed.
+//
+// Blocks whose columns and rows (with the halo) lie inside the buffer take
+// the unchecked path: uniform row addresses plus 32-bit lane offsets.
 
 constexpr int kSweepThreads = 256;
+constexpr int kSweepCols = kSweepThreads * 2;
 constexpr int kSweepRows = 64;
 
 struct HornArgs {
     float k[8];
     int32_t flags;
+    int32_t aligned;  // source and outputs allow 8-byte pair accesses at even offsets
 };
 
+// An input row as one lane holds it: its pair of cells, the cells left and
+// right of the pair, and the pair's horizontal differences (z3 - z1 for
+// each of the two columns), which three output rows share.
+struct Row {
+    float c0, c1, l, r, hd0, hd1;
+};
+
+// Correctly rounded sqrtf for x = +0 or x in [2^-96, FLT_MAX]: the hardware
+// root and the one-ulp correction the compiler's full sequence applies,
+// without its denormal scaling and special-value fix-up (not needed there).
+__device__ __forceinline__ bool sqrt_fast_ok(float x) { return x == 0.0f || (x >= 0x1p-96f && x <= 3.40282347e38f); }
+__device__ __forceinline__ float sqrt_fast(float x)
+{
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const float ydn = u2f(f2u(y) - 1u), yup = u2f(f2u(y) + 1u);
+    const float rdn = __builtin_fmaf(-ydn, y, x), rup = __builtin_fmaf(-yup, y, x);
+    float r = rdn <= 0.0f ? ydn : y;
+    return rup > 0.0f ? yup : r;
+}
+
+// Correctly rounded n / d for |n| in [2^-30, 2^30] and d in [1, 2^30]: the
+// compiler's refinement sequence (reciprocal refined by one Newton step, two
+// residual corrections) without the range scaling and fix-up, which never
+// apply there.
+__device__ __forceinline__ bool div_fast_ok(float n, float d)
+{
+    const float an = u2f(f2u(n) & 0x7fffffffu);
+    return an >= 0x1p-30f && an <= 0x1p30f && d >= 1.0f && d <= 0x1p30f;
+}
+__device__ __forceinline__ float div_fast(float n, float d)
+{
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = n * y;
+    float r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(r, y, q);
+}
+
+// Atan32 for x in [0, tan(pi/8)]: the reduction picks num = x, den = 1 and
+// y0 = 0 there, so t = x exactly, and y0 + v == v because v is never -0.
+__device__ __forceinline__ float atan32_small(float t)
+{
+    const float z = t * t;
+    float p = kAtanC4 * z - kAtanC3;
+    p = p * z + kAtanC2;
+    p = p * z - kAtanC1;
+    return (p * z) * t + t;
+}
+
+// Slope of a pair of cells; the fast paths are taken by whole waves.
+__device__ __forceinline__ void slope_pair(float gx0, float gy0, float gx1, float gy1, float scale, bool atan_,
+                                           float &o0, float &o1)
+{
+    const float s0 = gx0 * gx0 + gy0 * gy0, s1 = gx1 * gx1 + gy1 * gy1;
+    float m0, m1;
+    if (__all(sqrt_fast_ok(s0) && sqrt_fast_ok(s1))) {
+        m0 = sqrt_fast(s0);
+        m1 = sqrt_fast(s1);
+    } else {
+        m0 = __builtin_sqrtf(s0);
+        m1 = __builtin_sqrtf(s1);
+    }
+    if (!atan_) {
+        o0 = m0 * scale;
+        o1 = m1 * scale;
+    } else if (__all(m0 <= kTanPi8 && m1 <= kTanPi8)) {
+        o0 = atan32_small(m0) * scale;
+        o1 = atan32_small(m1) * scale;
+    } else {
+        o0 = atan32(m0) * scale;
+        o1 = atan32(m1) * scale;
+    }
+}
+
+// Hillshade of a pair (shade_of with the fast root and quotient when the
+// whole wave is in their ranges).
+__device__ __forceinline__ void shade_pair(float gx0, float gy0, float gx1, float gy1, float c, float bx, float by,
+                                           float &o0, float &o1)
+{
+    const float n0 = c + (bx * gx0 + by * gy0), n1 = c + (bx * gx1 + by * gy1);
+    const float w0 = 1.0f + (gx0 * gx0 + gy0 * gy0), w1 = 1.0f + (gx1 * gx1 + gy1 * gy1);
+    float v0, v1;
+    if (__all(sqrt_fast_ok(w0) && sqrt_fast_ok(w1))) {
+        const float d0 = sqrt_fast(w0), d1 = sqrt_fast(w1);
+        if (__all(div_fast_ok(n0, d0) && div_fast_ok(n1, d1))) {
+            v0 = div_fast(n0, d0);
+            v1 = div_fast(n1, d1);
+        } else {
+            v0 = n0 / d0;
+            v1 = n1 / d1;
+        }
+    } else {
+        v0 = n0 / __builtin_sqrtf(w0);
+        v1 = n1 / __builtin_sqrtf(w1);
+    }
+    if (v0 < 0.0f) v0 = 0.0f;
+    if (v0 > 255.0f) v0 = 255.0f;
+    if (v1 < 0.0f) v1 = 0.0f;
+    if (v1 > 255.0f) v1 = 255.0f;
+    o0 = v0;
+    o1 = v1;
+}
+
+// The op over the two cells of a lane from rows A (y-1), B (y), C (y+1):
+// outputs (o00, o01) for the first output and (o10, o11) for the second.
 template <int OP>
-__device__ __forceinline__ void horn_cell(const HornArgs &a, float z1, float z2, float z3, float z4, float z5,
-                                          float z6, float z7, float z8, float z9, float &o0, float &o1)
+__device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const Row &B, const Row &C, float &o00,
+                                          float &o01, float &o10, float &o11)
 {
     if constexpr (OP == STRATA_OP_CURVATURE) {
-        o0 = zt_curvature(z1, z2, z3, z4, z5, z6, z7, z8, z9, a.k, (a.flags >> 4) & 3);
+        const int kind = (a.flags >> 4) & 3;
+        o00 = zt_curvature(A.l, A.c0, A.c1, B.l, B.c0, B.c1, C.l, C.c0, C.c1, a.k, kind);
+        o01 = zt_curvature(A.c0, A.c1, A.r, B.c0, B.c1, B.r, C.c0, C.c1, C.r, a.k, kind);
     } else {
-        float gx = horn_dx(z1, z3, z4, z6, z7, z9) * a.k[0];
-        float gy = horn_dy(z1, z2, z3, z7, z8, z9) * a.k[1];
+        // Horn (internal/stencil/horn.go:172-180), grouped as the reference
+        // groups it; the differences are shared between rows and columns.
+        const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
+        const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
+        const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
+        const float dy0 = (gl + g1) + (g0 + g0);
+        const float dy1 = (g0 + gr) + (g1 + g1);
+        const float gx0 = dx0 * a.k[0], gy0 = dy0 * a.k[1];
+        const float gx1 = dx1 * a.k[0], gy1 = dy1 * a.k[1];
         if constexpr (OP == STRATA_OP_GRADIENT) {
-            o0 = gx;
-            o1 = gy;
+            o00 = gx0;
+            o01 = gx1;
+            o10 = gy0;
+            o11 = gy1;
         } else if constexpr (OP == STRATA_OP_SLOPE) {
-            o0 = slope_of(gx, gy, a.k[2], (a.flags & 1) != 0);
+            slope_pair(gx0, gy0, gx1, gy1, a.k[2], (a.flags & 1) != 0, o00, o01);
         } else if constexpr (OP == STRATA_OP_ASPECT) {
-            o0 = aspect_of(gx, gy, a.k[2], (a.flags & 2) != 0);
+            o00 = aspect_of(gx0, gy0, a.k[2], (a.flags & 2) != 0);
+            o01 = aspect_of(gx1, gy1, a.k[2], (a.flags & 2) != 0);
         } else {
-            o0 = shade_of(gx, gy, a.k[2], a.k[3], a.k[4]);
+            shade_pair(gx0, gy0, gx1, gy1, a.k[2], a.k[3], a.k[4], o00, o01);
+        }
+    }
+}
+
+template <int OP, bool SAFE>
+__device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
+                                      const DstView &d1, int64_t xb, int64_t y0, int64_t y1)
+{
+    constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
+    const int lane = threadIdx.x & 63;
+    const uint32_t off = 2u * threadIdx.x;  // the pair's first column, relative to xb
+    const int64_t x = xb + off;
+    const bool halo_lane = lane == 0 || lane == 63;
+    const int32_t hoff = lane == 0 ? (int32_t)off - 1 : (int32_t)off + 2;
+    const bool ci0 = x >= 1 && x < rg.W - 1;
+    const bool ci1 = x + 1 >= 1 && x + 1 < rg.W - 1;
+    const bool in0 = x >= rg.ox && x < rg.ox + rg.w;
+    const bool in1 = x + 1 >= rg.ox && x + 1 < rg.ox + rg.w;
+    const bool pairs = a.aligned != 0;
+
+    auto load = [&](Row &row, int64_t y) {
+        float c0, c1, h = 0.0f;
+        if constexpr (SAFE) {
+            const float *rp = s.p + (y - s.sy) * s.stride + (xb - s.sx);
+            if (pairs) {
+                const float2 v = *reinterpret_cast<const float2 *>(rp + off);
+                c0 = v.x;
+                c1 = v.y;
+            } else {
+                c0 = rp[off];
+                c1 = rp[off + 1];
+            }
+            if (halo_lane) h = rp[hoff];
+        } else {
+            c0 = load_cell(s, x, y);
+            c1 = load_cell(s, x + 1, y);
+            if (halo_lane) h = load_cell(s, xb + hoff, y);
+        }
+        row.c0 = c0;
+        row.c1 = c1;
+        row.l = h;  // the halo value until finish() forms l and r
+    };
+    auto finish = [&](Row &row) {
+        const float h = row.l;
+        row.l = from_left(row.c1, h);
+        row.r = from_right(row.c0, h);
+        row.hd0 = row.c1 - row.l;
+        row.hd1 = row.r - row.c0;
+    };
+    auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
+        float *rp = d.p + (y - rg.oy) * d.stride + (xb - rg.ox);
+        if (SAFE && pairs) {
+            *reinterpret_cast<float2 *>(rp + off) = make_float2(v0, v1);
+        } else {
+            if (in0) rp[off] = v0;
+            if (in1) rp[off + 1] = v1;
+        }
+    };
+
+    Row R[4];
+    load(R[0], y0 - 1);
+    load(R[1], y0);
+    load(R[2], y0 + 1);
+    finish(R[0]);
+    finish(R[1]);
+    for (int64_t y = y0; y < y1; y += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t yy = y + u;
+            if (yy < y1) {
+                Row &A = R[u & 3];
+                Row &B = R[(u + 1) & 3];
+                Row &C = R[(u + 2) & 3];
+                Row &D = R[(u + 3) & 3];
+                if (yy + 2 <= y1) load(D, yy + 2);
+                finish(C);
+                float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
+                horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
+                const bool ri = yy >= 1 && yy < rg.H - 1;
+                const float nan = edge_nan();
+                store(d0, yy, (ri && ci0) ? o00 : nan, (ri && ci1) ? o01 : nan);
+                if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
+            }
         }
     }
 }
@@ -91,44 +308,17 @@ template <int OP>
 __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region rg, SrcView s, DstView d0,
                                                             DstView d1)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t x = rg.ox + (int64_t)blockIdx.x * kSweepThreads + threadIdx.x;
-    const int64_t hx = lane == 0 ? x - 1 : x + 1;
-    const bool halo_lane = lane == 0 || lane == 63;
+    const int64_t xb = rg.ox + (int64_t)blockIdx.x * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)blockIdx.y * kSweepRows;
     const int64_t y1 = min(y0 + (int64_t)kSweepRows, rg.oy + rg.h);
-    const bool col_in = x >= rg.ox && x < rg.ox + rg.w;
-    const bool col_interior = x >= 1 && x < rg.W - 1;
-
-    // Rows y-1, y, y+1 of the window: centre value, left and right.
-    float cm = load_cell(s, x, y0 - 1);
-    float hm = halo_lane ? load_cell(s, hx, y0 - 1) : 0.0f;
-    float c0 = load_cell(s, x, y0);
-    float h0 = halo_lane ? load_cell(s, hx, y0) : 0.0f;
-    float cn = load_cell(s, x, y0 + 1);
-    float hn = halo_lane ? load_cell(s, hx, y0 + 1) : 0.0f;
-    float lm = from_left(cm, hm), rm = from_right(cm, hm);
-    float l0 = from_left(c0, h0), r0 = from_right(c0, h0);
-
-    for (int64_t y = y0; y < y1; ++y) {
-        const float cp = cn, hp = hn;
-        // Issue the next row's loads before this row's arithmetic.
-        if (y + 2 <= y1) {
-            cn = load_cell(s, x, y + 2);
-            hn = halo_lane ? load_cell(s, hx, y + 2) : 0.0f;
-        }
-        const float lp = from_left(cp, hp), rp = from_right(cp, hp);
-        float o0 = edge_nan(), o1 = edge_nan();
-        if (col_interior && y >= 1 && y < rg.H - 1) {
-            horn_cell<OP>(a, lm, cm, rm, l0, c0, r0, lp, cp, rp, o0, o1);
-        }
-        if (col_in) {
-            d0.p[(y - rg.oy) * d0.stride + (x - rg.ox)] = o0;
-            if constexpr (OP == STRATA_OP_GRADIENT) d1.p[(y - rg.oy) * d1.stride + (x - rg.ox)] = o1;
-        }
-        lm = l0; cm = c0; rm = r0;
-        l0 = lp; c0 = cp; r0 = rp;
-    }
+    // The block reads columns [xb-1, xb+kSweepCols] and rows [y0-1, y1] and
+    // writes columns [xb, xb+kSweepCols).
+    const bool safe = xb - 1 >= s.lx0 && xb + kSweepCols < s.lx1 && y0 - 1 >= s.ly0 && y1 < s.ly1 &&
+                      xb + kSweepCols <= rg.ox + rg.w;
+    if (safe)
+        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1);
+    else
+        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,47 +554,67 @@ __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, Src
 // validity: output bits, one 64-bit word per thread
 // ---------------------------------------------------------------------------
 
-// Bits [pos, pos+64) of m, reading only words up to index `last`.
-__device__ __forceinline__ uint64_t bits_at(const uint64_t *m, int64_t pos, int64_t last)
+// Bits [o, o+64) of the 192-bit little-endian value (w0, w1, w2), o < 128.
+__device__ __forceinline__ uint64_t bits3(uint64_t w0, uint64_t w1, uint64_t w2, int o)
 {
-    const int64_t w = pos >> 6;
-    const int sh = (int)(pos & 63);
-    uint64_t v = m[w] >> sh;
-    if (sh != 0 && w + 1 <= last) v |= m[w + 1] << (64 - sh);
-    return v;
+    if (o >= 64) {
+        o -= 64;
+        w0 = w1;
+        w1 = w2;
+    }
+    return o == 0 ? w0 : (w0 >> o) | (w1 << (64 - o));
 }
 
-__global__ __launch_bounds__(256) void validity_kernel(int r, Region rg, MaskView src, int has_src,
-                                                       MaskView dst, int64_t row0)
+// One thread per output mask word of one row of the region. For the
+// word's interior cells [lo, hi): each of the 2R+1 source rows is read
+// as the bits of cells [lo-R, hi+R) (at most three words, all loaded up
+// front), the rows are ANDed, and the result is eroded horizontally by
+// ANDing its 2R+1 shifts. Words only partly owned by the region's row
+// are updated with atomics on the row's own bits.
+template <int R>
+__global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, int has_src, MaskView dst,
+                                                       int64_t row0)
 {
-    const int64_t y = row0 + blockIdx.y;          // raster row
+    const int64_t y = row0 + blockIdx.y;
     const int64_t ob = dst.off + (y - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, y)
     const int64_t wfirst = ob >> 6, wlast = (ob + rg.w - 1) >> 6;
     const int64_t wi = wfirst + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (wi > wlast) return;
 
-    // Bits of this word that belong to the region's row.
     const int64_t wb = wi << 6;
     const int64_t own_lo = max(wb, ob), own_hi = min(wb + 64, ob + rg.w);
-    uint64_t own = (own_hi - own_lo == 64) ? ~0ull : (((1ull << (own_hi - own_lo)) - 1) << (own_lo - wb));
+    const uint64_t own = (own_hi - own_lo == 64) ? ~0ull : (((1ull << (own_hi - own_lo)) - 1) << (own_lo - wb));
 
-    // Interior cells of the row in this word.
     uint64_t val = 0;
-    if (y >= r && y < rg.H - r) {
-        const int64_t lo = max(own_lo, ob + (r - rg.ox));
-        const int64_t hi = min(own_hi, ob + (rg.W - r - rg.ox));
+    if (y >= R && y < rg.H - R) {
+        const int64_t lo = max(own_lo, ob + (R - rg.ox));
+        const int64_t hi = min(own_hi, ob + (rg.W - R - rg.ox));
         if (lo < hi) {
             const int n = (int)(hi - lo);
             uint64_t acc = (n == 64) ? ~0ull : ((1ull << n) - 1);
             if (has_src) {
-                const int64_t xlo = rg.ox + (lo - ob);  // first interior cell
-                for (int dy = -r; dy <= r; ++dy) {
-                    const int64_t a = src.off + (y + dy - src.y0) * src.stride + (xlo - r - src.x0);
-                    const int64_t last = (a + n + 2 * r - 1) >> 6;
-                    uint64_t rowv = ~0ull;
-                    for (int c = 0; c <= 2 * r; ++c) rowv &= bits_at(src.bits, a + c, last);
-                    acc &= rowv;
+                const int64_t xlo = rg.ox + (lo - ob);  // first interior cell of the word
+                uint64_t w[2 * R + 1][3];
+                int sh[2 * R + 1];
+#pragma unroll
+                for (int j = 0; j <= 2 * R; ++j) {
+                    const int64_t a = src.off + (y - R + j - src.y0) * src.stride + (xlo - R - src.x0);
+                    const int64_t base = a >> 6, last = (a + n + 2 * R - 1) >> 6;
+                    sh[j] = (int)(a & 63);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) w[j][k] = (base + k <= last) ? src.bits[base + k] : 0ull;
                 }
+                // Rows aligned to cell xlo-R: bits [0, 64) and [64, 64+2R).
+                uint64_t v0 = ~0ull, v1 = ~0ull;
+#pragma unroll
+                for (int j = 0; j <= 2 * R; ++j) {
+                    v0 &= bits3(w[j][0], w[j][1], w[j][2], sh[j]);
+                    if (R > 0) v1 &= bits3(w[j][0], w[j][1], w[j][2], sh[j] + 64);
+                }
+                uint64_t e = v0;
+#pragma unroll
+                for (int c = 1; c <= 2 * R; ++c) e &= (v0 >> c) | (v1 << (64 - c));
+                acc &= e;
             }
             val = acc << (lo - wb);
         }
@@ -433,7 +643,13 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
     for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
     a.flags = P.flags;
     const DstView d1 = nout > 1 ? d[1] : d[0];
-    const int64_t gx = cdiv(rg.w, kSweepThreads);
+    // Pair (8-byte) accesses need every pair's first column at an even
+    // offset from 8-byte-aligned row starts.
+    auto al8 = [](const void *p) { return ((uintptr_t)p & 7) == 0; };
+    bool aligned = al8(s.p) && (s.stride & 1) == 0 && ((rg.ox - s.sx) & 1) == 0;
+    for (int o = 0; o < nout; ++o) aligned = aligned && al8(d[o].p) && (d[o].stride & 1) == 0;
+    a.aligned = aligned ? 1 : 0;
+    const int64_t gx = cdiv(rg.w, kSweepCols);
     for (int64_t yb = 0; yb < cdiv(rg.h, kSweepRows); yb += kMaxGridY) {
         Region sub = rg;
         sub.oy = rg.oy + yb * kSweepRows;
@@ -533,8 +749,92 @@ hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has
     for (int64_t yb = 0; yb < rg.h; yb += kMaxGridY) {
         const int64_t rows = min(rg.h - yb, kMaxGridY);
         dim3 grid((unsigned)gx, (unsigned)rows);
-        hipLaunchKernelGGL(validity_kernel, grid, dim3(256), 0, st, r, rg, src, has_src, dst, rg.oy + yb);
+        switch (r) {
+#define STRATA_VALIDITY_CASE(RR)                                                                            \
+    case RR:                                                                                                \
+        hipLaunchKernelGGL(validity_kernel<RR>, grid, dim3(256), 0, st, rg, src, has_src, dst, rg.oy + yb); \
+        break;
+            STRATA_VALIDITY_CASE(0)
+            STRATA_VALIDITY_CASE(1)
+            STRATA_VALIDITY_CASE(2)
+            STRATA_VALIDITY_CASE(3)
+            STRATA_VALIDITY_CASE(4)
+            STRATA_VALIDITY_CASE(5)
+            STRATA_VALIDITY_CASE(6)
+            STRATA_VALIDITY_CASE(7)
+            STRATA_VALIDITY_CASE(8)
+#undef STRATA_VALIDITY_CASE
+        default:
+            return hipErrorInvalidValue;
+        }
     }
+    return hipGetLastError();
+}
+
+} // namespace strata
+
+// ---------------------------------------------------------------------------
+// Self-checks of the fast paths against the full sequences (run by tests)
+// ---------------------------------------------------------------------------
+
+namespace strata {
+
+__device__ __forceinline__ uint32_t mix32(uint64_t v)
+{
+    v ^= v >> 33;
+    v *= 0xff51afd7ed558ccdull;
+    v ^= v >> 33;
+    v *= 0xc4ceb9fe1a85ec53ull;
+    v ^= v >> 33;
+    return (uint32_t)v;
+}
+
+// kind 0: sqrt_fast vs sqrtf over bit patterns [lo, lo+n) in its range.
+// kind 1: atan32_small vs atan32 over bit patterns [lo, lo+n) up to tan(pi/8).
+// kind 2: div_fast vs '/' over n hashed pairs (seed lo) in its range.
+__global__ void selfcheck_kernel(int kind, uint64_t lo, uint64_t n, unsigned long long *bad,
+                                 unsigned long long *first, unsigned long long *checked)
+{
+    unsigned long long mine = 0, seen = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        bool differ = false, ran = false;
+        if (kind == 0) {
+            const float x = u2f((uint32_t)(lo + i));
+            if (sqrt_fast_ok(x)) {
+                ran = true;
+                differ = f2u(sqrt_fast(x)) != f2u(__builtin_sqrtf(x));
+            }
+        } else if (kind == 1) {
+            const float x = u2f((uint32_t)(lo + i));
+            if (x >= 0.0f && x <= kTanPi8) {
+                ran = true;
+                differ = f2u(atan32_small(x)) != f2u(atan32(x));
+            }
+        } else {
+            // |n| in [2^-30, 2^30) with random sign and mantissa, d in [1, 2^30).
+            const uint32_t h1 = mix32(lo * 0x9e3779b97f4a7c15ull + 2 * i), h2 = mix32(lo + 2 * i + 1);
+            const uint32_t en = 127 - 30 + (h1 >> 26) % 60, ed = 127 + (h2 >> 27) % 30;
+            const float num = u2f((h1 & 0x80000000u) | (en << 23) | (h1 & 0x7fffffu));
+            const float den = u2f((ed << 23) | (h2 & 0x7fffffu));
+            if (div_fast_ok(num, den)) {
+                ran = true;
+                differ = f2u(div_fast(num, den)) != f2u(num / den);
+            }
+        }
+        seen += ran;
+        if (differ) {
+            ++mine;
+            atomicMin(first, (unsigned long long)i);
+        }
+    }
+    if (mine) atomicAdd(bad, mine);
+    atomicAdd(checked, seen);
+}
+
+hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long long *dev3, hipStream_t st)
+{
+    hipLaunchKernelGGL(selfcheck_kernel, dim3(8192), dim3(256), 0, st, kind, lo, n, dev3, dev3 + 1, dev3 + 2);
     return hipGetLastError();
 }
 

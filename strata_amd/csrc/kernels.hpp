@@ -63,4 +63,8 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
 hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has_src, const MaskView &dst,
                            hipStream_t st);
 
+// Self-check of a fast path against the full sequence over n inputs (see
+// selfcheck_kernel); dev3 holds {mismatches, first mismatch index, checked}.
+hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long long *dev3, hipStream_t st);
+
 } // namespace strata

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -365,6 +366,24 @@ struct Operands {
     int64_t W, H;
 };
 
+// Kernel timing for measurement: when enabled, each data and validity
+// launch is bracketed by HIP events on its own stream and the elapsed times
+// are summed when the profile is read.
+struct Timing {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> data, validity;
+};
+Timing g_timing;
+std::mutex g_timing_mu;
+
+hipEvent_t timing_event(hipStream_t st)
+{
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    (void)hipEventRecord(e, st);
+    return e;
+}
+
 int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st)
 {
     Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
@@ -375,13 +394,24 @@ int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st
         d[o].p = op.dst[o]->data + (rc.y0 - op.dy[o]) * op.dst[o]->stride + (rc.x0 - op.dx[o]);
         d[o].stride = op.dst[o]->stride;
     }
+    const bool timed = g_timing.on;
+    hipEvent_t e0 = timed ? timing_event(st) : nullptr;
     HIP_TRY(launch_data(P, rg, s, d, op.nout, st), "launching the data kernel");
+    if (timed) {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        g_timing.data.emplace_back(e0, timing_event(st));
+    }
     for (int o = 0; o < op.nout; ++o) {
         if (!op.dst[o]->valid) continue;
         const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o]);
         const bool has_src = op.src->valid != nullptr;
         const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy) : dm;
+        hipEvent_t v0 = timed ? timing_event(st) : nullptr;
         HIP_TRY(launch_validity(P.radius, rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");
+        if (timed) {
+            std::lock_guard<std::mutex> lk(g_timing_mu);
+            g_timing.validity.emplace_back(v0, timing_event(st));
+        }
     }
     return STRATA_OK;
 }
@@ -452,6 +482,59 @@ int strata_init(int device)
 const char *strata_last_error(void) { return g_err.c_str(); }
 
 const char *strata_version(void) { return "strata-hip 0.1 gfx950"; }
+
+int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, uint64_t *first, uint64_t *checked)
+{
+    if (kind < 0 || kind > 2) return fail(STRATA_EINVAL, "strata: unknown self-check %d", kind);
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, 3 * sizeof(unsigned long long)), "self-check buffer");
+    unsigned long long h[3] = {0, ~0ull, 0};
+    hipError_t e = hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_selfcheck(kind, lo, n, d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "self-check");
+    if (mismatches) *mismatches = h[0];
+    if (first) *first = h[1];
+    if (checked) *checked = h[2];
+    g_err.clear();
+    return STRATA_OK;
+}
+
+int strata_timing_begin(void)
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    for (auto &p : g_timing.data) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+    for (auto &p : g_timing.validity) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+    g_timing.data.clear();
+    g_timing.validity.clear();
+    g_timing.on = true;
+    return STRATA_OK;
+}
+
+int strata_timing_end(double *data_ms, int64_t *data_launches, double *validity_ms, int64_t *validity_launches)
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing.on = false;
+    double dms = 0, vms = 0;
+    for (auto &p : g_timing.data) {
+        float ms = 0;
+        HIP_TRY(hipEventSynchronize(p.second), "timing");
+        HIP_TRY(hipEventElapsedTime(&ms, p.first, p.second), "timing");
+        dms += ms;
+    }
+    for (auto &p : g_timing.validity) {
+        float ms = 0;
+        HIP_TRY(hipEventSynchronize(p.second), "timing");
+        HIP_TRY(hipEventElapsedTime(&ms, p.first, p.second), "timing");
+        vms += ms;
+    }
+    if (data_ms) *data_ms = dms;
+    if (data_launches) *data_launches = (int64_t)g_timing.data.size();
+    if (validity_ms) *validity_ms = vms;
+    if (validity_launches) *validity_launches = (int64_t)g_timing.validity.size();
+    return STRATA_OK;
+}
 
 int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
                       const strata_options *opts, int chunked, strata_stats *stats)
@@ -633,6 +716,31 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
             if (!dst[o].resident && (rc = alloc_buf(bufs[nsrc + o], g.tileW, g.tileH, dst[o].masked))) return rc;
 
         strata_stats s{};
+        bool all_resident = true;
+        for (int i = 0; i < nsrc; ++i) all_resident = all_resident && src[i].resident;
+        for (int o = 0; o < ndst; ++o) all_resident = all_resident && dst[o].resident;
+        if (all_resident && !opts_cancel) {
+            // Device tile queue over resident operands with nothing to poll:
+            // every tile reads its halo straight from HBM and writes in place,
+            // so the queue is one launch over the raster (tiling never changes
+            // a result) and the tiles' Stats follow in closed form.
+            strata_raster sv = *src[0].resident, dv[2]{};
+            if (sv.mem == STRATA_MEM_HOST) return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+            if (!src[0].masked) sv.valid = nullptr;
+            for (int o = 0; o < ndst; ++o) {
+                dv[o] = *dst[o].resident;
+                if (dv[o].mem == STRATA_MEM_HOST)
+                    return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                if (!dst[o].masked) dv[o].valid = nullptr;
+            }
+            Operands op{&sv, 0, 0, {&dv[0], &dv[ndst > 1 ? 1 : 0]}, {0, 0}, {0, 0}, ndst, W, H};
+            if ((rc = run_rect(P, op, Rect{0, 0, W, H}, st))) return rc;
+            for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, g, nsrc, ndst, tile_rect(g, t));
+            if (!(opts && opts->async)) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+            add_stats(stats, s, nsrc, ndst);
+            g_err.clear();
+            return STRATA_OK;
+        }
         for (int64_t t = 0; t < g.tiles; ++t) {
             if (cancelled(opts)) {
                 status = STRATA_CANCELLED;

@@ -468,7 +468,7 @@ def _resident(x) -> Optional[Float32Raster]:
 
 
 def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[RasterSource], k: KernelDesc,
-                   opts: Optional[Options] = None) -> Optional[Exception]:
+                   opts: Optional[Options] = None, *, async_: bool = False) -> Optional[Exception]:
     """exec.ProcessChunked: tiles read with their halo, computed on the
     device, written to the sinks; a memory source or sink in device memory
     is read or written in place (the device tile queue)."""
@@ -583,7 +583,7 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
             c.write = fn
     kc, taps = k.to_c()
     rasters = [x.r for x in list(src) + list(dst) if isinstance(x, (MemorySource, MemorySink))]
-    co = _c_options(opts, ctx, _stream_handle(rasters))
+    co = _c_options(opts, ctx if not async_ else None, _stream_handle(rasters), async_)
     st = _lib.Stats()
     rc = _lib.lib().strata_process_chunked(ctypes.byref(kc), cdst, len(dst), csrc, len(src), ctypes.byref(co),
                                            ctypes.byref(st))

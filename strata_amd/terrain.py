@@ -38,7 +38,10 @@ class TerrainPanic(EnginePanic):
 
 
 def _f32(x: float) -> float:
-    return float(np.float32(x))
+    """float32(x) as Go rounds it: overflow gives ±Inf, which the usable
+    checks then reject."""
+    with np.errstate(over="ignore"):
+        return float(np.float32(x))
 
 
 def _usable(k: float) -> bool:
