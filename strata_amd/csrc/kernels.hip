@@ -221,40 +221,41 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
     }
 }
 
-template <int OP, bool SAFE>
+template <int OP>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
                                       const DstView &d1, int64_t xb, int64_t y0, int64_t y1)
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
     const int lane = threadIdx.x & 63;
-    const uint32_t off = 2u * threadIdx.x;  // the pair's first column, relative to xb
+    const int32_t off = 2 * (int32_t)threadIdx.x;  // the pair's first column, relative to xb
     const int64_t x = xb + off;
     const bool halo_lane = lane == 0 || lane == 63;
-    const int32_t hoff = lane == 0 ? (int32_t)off - 1 : (int32_t)off + 2;
+    // Column offsets of the cells this lane loads, clamped into the buffer:
+    // a clamped cell is read only for edge outputs, which become NaN.
+    auto col = [&](int64_t c) { return (int32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
+    const int32_t o0 = col(x), o1 = col(x + 1), oh = col(lane == 0 ? x - 1 : x + 2);
     const bool ci0 = x >= 1 && x < rg.W - 1;
     const bool ci1 = x + 1 >= 1 && x + 1 < rg.W - 1;
     const bool in0 = x >= rg.ox && x < rg.ox + rg.w;
     const bool in1 = x + 1 >= rg.ox && x + 1 < rg.ox + rg.w;
-    const bool pairs = a.aligned != 0;
+    // Block-uniform: every pair is inside the buffer (loads) and the region
+    // (stores), at 8-byte-aligned addresses.
+    const bool pair_in = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1;
+    const bool pair_out = a.aligned && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
+    const int32_t poff = (int32_t)(xb - s.sx) + off;
 
     auto load = [&](Row &row, int64_t y) {
+        const float *rp = s.p + (min(max(y, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
         float c0, c1, h = 0.0f;
-        if constexpr (SAFE) {
-            const float *rp = s.p + (y - s.sy) * s.stride + (xb - s.sx);
-            if (pairs) {
-                const float2 v = *reinterpret_cast<const float2 *>(rp + off);
-                c0 = v.x;
-                c1 = v.y;
-            } else {
-                c0 = rp[off];
-                c1 = rp[off + 1];
-            }
-            if (halo_lane) h = rp[hoff];
+        if (pair_in) {
+            const float2 v = *reinterpret_cast<const float2 *>(rp + poff);
+            c0 = v.x;
+            c1 = v.y;
         } else {
-            c0 = load_cell(s, x, y);
-            c1 = load_cell(s, x + 1, y);
-            if (halo_lane) h = load_cell(s, xb + hoff, y);
+            c0 = rp[o0];
+            c1 = rp[o1];
         }
+        if (halo_lane) h = rp[oh];
         row.c0 = c0;
         row.c1 = c1;
         row.l = h;  // the halo value until finish() forms l and r
@@ -268,7 +269,7 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     };
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
         float *rp = d.p + (y - rg.oy) * d.stride + (xb - rg.ox);
-        if (SAFE && pairs) {
+        if (pair_out) {
             *reinterpret_cast<float2 *>(rp + off) = make_float2(v0, v1);
         } else {
             if (in0) rp[off] = v0;
@@ -311,14 +312,7 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     const int64_t xb = rg.ox + (int64_t)blockIdx.x * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)blockIdx.y * kSweepRows;
     const int64_t y1 = min(y0 + (int64_t)kSweepRows, rg.oy + rg.h);
-    // The block reads columns [xb-1, xb+kSweepCols] and rows [y0-1, y1] and
-    // writes columns [xb, xb+kSweepCols).
-    const bool safe = xb - 1 >= s.lx0 && xb + kSweepCols < s.lx1 && y0 - 1 >= s.ly0 && y1 < s.ly1 &&
-                      xb + kSweepCols <= rg.ox + rg.w;
-    if (safe)
-        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1);
-    else
-        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1);
+    sweep<OP>(a, rg, s, d0, d1, xb, y0, y1);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,21 +334,44 @@ enum FocalKind {
     FK_FIT = 5,
 };
 
+// The op's constants; taps come through an accessor (kernel arguments for a
+// standalone launch, a device buffer for a batch).
+struct FocalArgs {
+    int32_t op, flags;
+    float k[8];
+};
+struct ArgTaps {
+    const Params &P;
+    __device__ float operator[](int i) const { return P.taps[i]; }
+};
+struct PtrTaps {
+    const float *p;
+    __device__ float operator[](int i) const { return p[i]; }
+};
+
+// LDS floats one focal_block<KIND, R> uses.
 template <int KIND, int R>
-__global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Region rg, SrcView s, DstView d0,
-                                                           DstView d1)
+constexpr int focal_lds_floats()
+{
+    return (kTileY + 2 * R) * (kTileX + 2 * R) +
+           ((KIND == FK_CORRELATE) ? 0 : (KIND == FK_FIT ? 3 : 1) * kTileY * (kTileX + 2 * R));
+}
+
+// One kTileX x kTileY output tile at (tx0, ty0) of the region.
+template <int KIND, int R, class TAPS>
+__device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps, const Region &rg,
+                                            const SrcView &s, const DstView &d0, const DstView &d1, int64_t tx0,
+                                            int64_t ty0, float *lds)
 {
     constexpr int N = 2 * R + 1;
     constexpr int LW = kTileX + 2 * R;  // staged columns
     constexpr int LH = kTileY + 2 * R;  // staged rows
-    constexpr int NCOL = (KIND == FK_CORRELATE) ? 1 : (KIND == FK_FIT ? 3 : 1);
-    __shared__ float tile[LH][LW];
-    __shared__ float colv[NCOL][(KIND == FK_CORRELATE) ? 1 : kTileY][LW];
+    constexpr int CY = (KIND == FK_CORRELATE) ? 1 : kTileY;
+    float(*tile)[LW] = reinterpret_cast<float(*)[LW]>(lds);
+    float(*colv)[CY][LW] = reinterpret_cast<float(*)[CY][LW]>(lds + LH * LW);
 
     const int tid = threadIdx.x;
-    const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kTileX;
-    const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kTileY;
-
+    __syncthreads();  // the LDS may still be read by the previous tile of a batch
     for (int i = tid; i < LH * LW; i += kTileThreads) {
         int row = i / LW, col = i - row * LW;
         tile[row][col] = load_cell(s, tx0 - R + col, ty0 - R + row);
@@ -382,7 +399,7 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
                 for (int o = 0; o < kRowsPerWave; ++o) {
                     const int j = row - o;
                     if (j >= 0 && j < N) {
-                        const float term = P.taps[j * N + c] * v;
+                        const float term = taps[j * N + c] * v;
                         acc[o] = (j == 0 && c == 0) ? term : acc[o] + term;
                     }
                 }
@@ -396,10 +413,9 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
             const int rr = i / LW, cc = i - rr * LW;
             if constexpr (KIND == FK_SEPARABLE) {
                 // ColumnCorrelate with the col taps (focalrow.go:232-242).
-                const float *col = P.taps + N;
-                float a = col[0] * tile[rr][cc];
+                float a = taps[N] * tile[rr][cc];
 #pragma unroll
-                for (int j = 1; j < N; ++j) a = a + col[j] * tile[rr + j][cc];
+                for (int j = 1; j < N; ++j) a = a + taps[N + j] * tile[rr + j][cc];
                 colv[0][rr][cc] = a;
             } else if constexpr (KIND == FK_MEAN) {
                 float a = tile[rr][cc];  // ColumnSum (focalrow.go:244-250)
@@ -442,10 +458,10 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
             const int rr = rbase + o;
             out1[o] = 0.0f;
             if constexpr (KIND == FK_SEPARABLE) {
-                const float *row = P.taps;  // RowCorrelate (focalrow.go:275-284)
-                float a = row[0] * colv[0][rr][lane];
+                // RowCorrelate with the row taps (focalrow.go:275-284).
+                float a = taps[0] * colv[0][rr][lane];
 #pragma unroll
-                for (int c = 1; c < N; ++c) a = a + row[c] * colv[0][rr][lane + c];
+                for (int c = 1; c < N; ++c) a = a + taps[c] * colv[0][rr][lane + c];
                 out0[o] = a;
             } else if constexpr (KIND == FK_MEAN) {
                 float a = colv[0][rr][lane];  // RowMean (focalrow.go:286-293)
@@ -521,6 +537,21 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
                 d1.p[(y - rg.oy) * d1.stride + (x - rg.ox)] = interior ? out1[o] : edge_nan();
         }
     }
+}
+
+template <int KIND, int R>
+__global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Region rg, SrcView s, DstView d0,
+                                                           DstView d1)
+{
+    __shared__ float lds[focal_lds_floats<KIND, R>()];
+    FocalArgs a;
+    a.op = P.op;
+    a.flags = P.flags;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
+    const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kTileX;
+    const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kTileY;
+    focal_block<KIND, R>(a, ArgTaps{P}, rg, s, d0, d1, tx0, ty0, lds);
 }
 
 // ---------------------------------------------------------------------------
