@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=2000, help="rows in the CPU baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work (thread-seconds) to aim for")
+    ap.add_argument("--sweep-rows", type=int, default=0,
+                    help="rows per block of the radius-1 kernels (0: the library's default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     return ap.parse_args()
@@ -72,6 +74,11 @@ def main() -> int:
     dev = torch.device("cuda:0")
     L = _lib.lib()
     assert L.strata_init(0) == 0, _lib.last_error()
+    tuning = _lib.Tuning()
+    L.strata_get_tuning(tuning)
+    if args.sweep_rows:
+        tuning.sweep_rows = args.sweep_rows
+        assert L.strata_set_tuning(tuning) == 0, _lib.last_error()
     N = args.size
     cells = N * N
 
@@ -180,7 +187,7 @@ def main() -> int:
                  f"{covered * 100:.2f}% NoData in circular blobs of radius 5-50"),
         "config": {"workload": f"terrain.SlopeChunked {N}x{N} float32 DEM, device-resident, "
                                f"TileHeight {args.tile_height}, Horn 3x3, CellSize 30, degrees",
-                   "width": N, "height": N, "tile_height": args.tile_height,
+                   "width": N, "height": N, "tile_height": args.tile_height, "sweep_rows": tuning.sweep_rows,
                    "stats": {"cells": st.Cells, "tiles": st.Tiles, "bands": st.Bands,
                              "amplification": round(st.Amplification(), 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

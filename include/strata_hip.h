@@ -226,13 +226,27 @@ int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width,
                       const strata_options *opts, int chunked, strata_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Launch tuning (no counterpart in the reference): shapes only, never  */
+/* results                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct strata_tuning {
+    int32_t sweep_rows;  /* rows per block of the radius-1 terrain kernels (4..4096) */
+    int32_t reserved[7];
+} strata_tuning;
+
+int strata_get_tuning(strata_tuning *t);
+int strata_set_tuning(const strata_tuning *t);
+
+/* ------------------------------------------------------------------ */
 /* Self-checks (no counterpart in the reference)                       */
 /* ------------------------------------------------------------------ */
 
 /* Compare a device fast path with the full correctly rounded sequence it
  * stands in for: kind 0 the square root over the float bit patterns
  * [lo, lo+n), kind 1 the small-argument arctangent over [lo, lo+n),
- * kind 2 the quotient over n hashed operand pairs (seed lo). Reports the
+ * kind 2 the quotient over n hashed operand pairs (seed lo), kind 3 the
+ * fused small-slope path (root and arctangent of s = gx^2 + gy^2) over the
+ * bit patterns [lo, lo+n). Reports the
  * number of differing results, the first differing index and how many
  * inputs were in the fast path's range. */
 int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, uint64_t *first, uint64_t *checked);

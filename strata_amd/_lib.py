@@ -96,6 +96,10 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("sweep_rows", c_int32), ("reserved", c_int32 * 7)]
+
+
 READ_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
                            c_int64)
 WRITE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
@@ -134,6 +138,8 @@ EXPORTS = (
     "strata_process",
     "strata_process_chunked",
     "strata_plan_stats",
+    "strata_get_tuning",
+    "strata_set_tuning",
     "strata_selfcheck",
     "strata_timing_begin",
     "strata_timing_end",
@@ -167,6 +173,10 @@ def lib() -> ctypes.CDLL:
     L.strata_plan_stats.argtypes = [POINTER(Kernel), c_int, c_int, c_int64, c_int64, POINTER(Options), c_int,
                                     POINTER(Stats)]
     L.strata_plan_stats.restype = c_int
+    L.strata_get_tuning.argtypes = [POINTER(Tuning)]
+    L.strata_get_tuning.restype = c_int
+    L.strata_set_tuning.argtypes = [POINTER(Tuning)]
+    L.strata_set_tuning.restype = c_int
     L.strata_selfcheck.argtypes = [c_int, ctypes.c_uint64, ctypes.c_uint64, POINTER(ctypes.c_uint64),
                                    POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64)]
     L.strata_selfcheck.restype = c_int

@@ -14,11 +14,11 @@
 //                and its halo are staged in LDS; separable forms run the
 //                column pass into LDS and the row pass from it.
 //   point_rows   radius-0 transfer ops (Lookup, Reclass).
-//   validity     output validity, word by word: AND of the (2r+1) source
-//                rows, horizontal erosion by shifts, interior mask; whole
+//   validity     output validity, one 32-bit mask word per lane: AND of the
+//                (2r+1) source rows, horizontal erosion by funnel shifts
+//                with the neighbouring lanes' words, interior mask; whole
 //                words are stored, words shared with cells outside the
-//                region are updated with 64-bit atomics on the region's
-//                own bits only.
+//                region are updated with atomics on the region's own bits.
 //
 // The arithmetic is arith.hpp's, in the reference's order.
 #include "arith.hpp"
@@ -77,6 +77,8 @@ struct HornArgs {
     float k[8];
     int32_t flags;
     int32_t aligned;  // source and outputs allow 8-byte pair accesses at even offsets
+    int32_t rows;     // rows per strip (one block's height)
+    uint32_t blocks_x;  // column blocks per strip
 };
 
 // An input row as one lane holds it: its pair of cells, the cells left and
@@ -89,7 +91,11 @@ struct Row {
 // Correctly rounded sqrtf for x = +0 or x in [2^-96, FLT_MAX]: the hardware
 // root and the one-ulp correction the compiler's full sequence applies,
 // without its denormal scaling and special-value fix-up (not needed there).
-__device__ __forceinline__ bool sqrt_fast_ok(float x) { return x == 0.0f || (x >= 0x1p-96f && x <= 3.40282347e38f); }
+__device__ __forceinline__ bool sqrt_fast_ok(float x)
+{
+    const uint32_t u = f2u(x);  // +0, or [2^-96, FLT_MAX] (0x0f800000 .. 0x7f7fffff)
+    return (u == 0u) | ((u - 0x0f800000u) <= (0x7f7fffffu - 0x0f800000u));
+}
 __device__ __forceinline__ float sqrt_fast(float x)
 {
     const float y = __builtin_amdgcn_sqrtf(x);
@@ -106,7 +112,7 @@ __device__ __forceinline__ float sqrt_fast(float x)
 __device__ __forceinline__ bool div_fast_ok(float n, float d)
 {
     const float an = u2f(f2u(n) & 0x7fffffffu);
-    return an >= 0x1p-30f && an <= 0x1p30f && d >= 1.0f && d <= 0x1p30f;
+    return (an >= 0x1p-30f) & (an <= 0x1p30f) & (d >= 1.0f) & (d <= 0x1p30f);
 }
 __device__ __forceinline__ float div_fast(float n, float d)
 {
@@ -131,13 +137,32 @@ __device__ __forceinline__ float atan32_small(float t)
     return (p * z) * t + t;
 }
 
+// True on every active lane (a wave-uniform vote without exec branching).
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+
+// s = gx^2 + gy^2 whose rounded root m satisfies both fast paths: s = +0 or
+// s in [2^-96, S*], where S* (0x3e2fb0cd) is the largest float whose
+// correctly rounded root is <= tan(pi/8) as float32. Then m = sqrt_fast(s)
+// and Atan32(m) = atan32_small(m).
+constexpr uint32_t kSlopeSmallMax = 0x3e2fb0cdu;
+__device__ __forceinline__ bool slope_small_ok(float s)
+{
+    const uint32_t u = f2u(s);
+    return (u == 0u) | ((u - 0x0f800000u) <= (kSlopeSmallMax - 0x0f800000u));
+}
+
 // Slope of a pair of cells; the fast paths are taken by whole waves.
 __device__ __forceinline__ void slope_pair(float gx0, float gy0, float gx1, float gy1, float scale, bool atan_,
                                            float &o0, float &o1)
 {
     const float s0 = gx0 * gx0 + gy0 * gy0, s1 = gx1 * gx1 + gy1 * gy1;
+    if (atan_ && wave_all(slope_small_ok(s0) & slope_small_ok(s1))) {
+        o0 = atan32_small(sqrt_fast(s0)) * scale;
+        o1 = atan32_small(sqrt_fast(s1)) * scale;
+        return;
+    }
     float m0, m1;
-    if (__all(sqrt_fast_ok(s0) && sqrt_fast_ok(s1))) {
+    if (wave_all(sqrt_fast_ok(s0) & sqrt_fast_ok(s1))) {
         m0 = sqrt_fast(s0);
         m1 = sqrt_fast(s1);
     } else {
@@ -147,7 +172,7 @@ __device__ __forceinline__ void slope_pair(float gx0, float gy0, float gx1, floa
     if (!atan_) {
         o0 = m0 * scale;
         o1 = m1 * scale;
-    } else if (__all(m0 <= kTanPi8 && m1 <= kTanPi8)) {
+    } else if (wave_all((m0 <= kTanPi8) & (m1 <= kTanPi8))) {
         o0 = atan32_small(m0) * scale;
         o1 = atan32_small(m1) * scale;
     } else {
@@ -164,9 +189,9 @@ __device__ __forceinline__ void shade_pair(float gx0, float gy0, float gx1, floa
     const float n0 = c + (bx * gx0 + by * gy0), n1 = c + (bx * gx1 + by * gy1);
     const float w0 = 1.0f + (gx0 * gx0 + gy0 * gy0), w1 = 1.0f + (gx1 * gx1 + gy1 * gy1);
     float v0, v1;
-    if (__all(sqrt_fast_ok(w0) && sqrt_fast_ok(w1))) {
+    if (wave_all(sqrt_fast_ok(w0) & sqrt_fast_ok(w1))) {
         const float d0 = sqrt_fast(w0), d1 = sqrt_fast(w1);
-        if (__all(div_fast_ok(n0, d0) && div_fast_ok(n1, d1))) {
+        if (wave_all(div_fast_ok(n0, d0) & div_fast_ok(n1, d1))) {
             v0 = div_fast(n0, d0);
             v1 = div_fast(n1, d1);
         } else {
@@ -221,34 +246,33 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
     }
 }
 
-template <int OP>
+// PAIRS: every pair of the block lies inside the buffer and the region, at
+// 8-byte-aligned addresses, so rows move as dwordx2 loads and stores.
+template <int OP, bool PAIRS>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
                                       const DstView &d1, int64_t xb, int64_t y0, int64_t y1)
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
     const int lane = threadIdx.x & 63;
-    const int32_t off = 2 * (int32_t)threadIdx.x;  // the pair's first column, relative to xb
+    const uint32_t off = 2u * threadIdx.x;  // the pair's first column, relative to xb
     const int64_t x = xb + off;
     const bool halo_lane = lane == 0 || lane == 63;
-    // Column offsets of the cells this lane loads, clamped into the buffer:
-    // a clamped cell is read only for edge outputs, which become NaN.
-    auto col = [&](int64_t c) { return (int32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
-    const int32_t o0 = col(x), o1 = col(x + 1), oh = col(lane == 0 ? x - 1 : x + 2);
+    // Offsets from a row's start of the cells this lane loads, clamped into
+    // the buffer: a clamped cell is read only for edge outputs, which become
+    // NaN. They are never negative, since the buffer starts at or after sx.
+    auto col = [&](int64_t c) { return (uint32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
+    const uint32_t o0 = col(x), o1 = col(x + 1), oh = col(lane == 0 ? x - 1 : x + 2);
+    const uint32_t so = (uint32_t)(xb - rg.ox) + off;  // store offset from a region row's start
     const bool ci0 = x >= 1 && x < rg.W - 1;
     const bool ci1 = x + 1 >= 1 && x + 1 < rg.W - 1;
     const bool in0 = x >= rg.ox && x < rg.ox + rg.w;
     const bool in1 = x + 1 >= rg.ox && x + 1 < rg.ox + rg.w;
-    // Block-uniform: every pair is inside the buffer (loads) and the region
-    // (stores), at 8-byte-aligned addresses.
-    const bool pair_in = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1;
-    const bool pair_out = a.aligned && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
-    const int32_t poff = (int32_t)(xb - s.sx) + off;
 
     auto load = [&](Row &row, int64_t y) {
         const float *rp = s.p + (min(max(y, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
         float c0, c1, h = 0.0f;
-        if (pair_in) {
-            const float2 v = *reinterpret_cast<const float2 *>(rp + poff);
+        if constexpr (PAIRS) {
+            const float2 v = *reinterpret_cast<const float2 *>(rp + o0);
             c0 = v.x;
             c1 = v.y;
         } else {
@@ -268,12 +292,12 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
         row.hd1 = row.r - row.c0;
     };
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
-        float *rp = d.p + (y - rg.oy) * d.stride + (xb - rg.ox);
-        if (pair_out) {
-            *reinterpret_cast<float2 *>(rp + off) = make_float2(v0, v1);
+        float *rp = d.p + (y - rg.oy) * d.stride;
+        if constexpr (PAIRS) {
+            *reinterpret_cast<float2 *>(rp + so) = make_float2(v0, v1);
         } else {
-            if (in0) rp[off] = v0;
-            if (in1) rp[off + 1] = v1;
+            if (in0) rp[so] = v0;
+            if (in1) rp[so + 1] = v1;
         }
     };
 
@@ -309,10 +333,24 @@ template <int OP>
 __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region rg, SrcView s, DstView d0,
                                                             DstView d1)
 {
-    const int64_t xb = rg.ox + (int64_t)blockIdx.x * kSweepCols;
-    const int64_t y0 = rg.oy + (int64_t)blockIdx.y * kSweepRows;
-    const int64_t y1 = min(y0 + (int64_t)kSweepRows, rg.oy + rg.h);
-    sweep<OP>(a, rg, s, d0, d1, xb, y0, y1);
+    // Workgroup ids are dealt round-robin to the chip's XCDs, each with its
+    // own L2. Renumber so each XCD takes a contiguous run of tiles in raster
+    // order: side-by-side tiles share the cache lines of their halo columns,
+    // and tiles one above the other their halo rows, which then stay in the
+    // same L2 instead of being fetched from HBM once per XCD.
+    constexpr uint32_t kXcds = 8;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
+    const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
+    const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
+    const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
+                       xb + kSweepCols <= rg.ox + rg.w;
+    if (pairs)
+        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1);
+    else
+        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1);
 }
 
 // ---------------------------------------------------------------------------
@@ -582,81 +620,184 @@ __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, Src
 }
 
 // ---------------------------------------------------------------------------
-// validity: output bits, one 64-bit word per thread
+// validity: output bits, one 32-bit mask word per lane
 // ---------------------------------------------------------------------------
+//
+// Masks are LSB-first, so a []uint64 bitmap read as 32-bit words has bit b in
+// word b>>5 at position b&31, and every funnel shift of two 32-bit words is
+// one v_alignbit.
+//
+// A wave takes kVsRows rows of a region and 64 consecutive 32-bit "cell
+// words" of them: lane L holds cells [xc + 32L, xc + 32L + 32) of each row.
+// It first loads, one coalesced word per lane, every source row the strip
+// needs (kVsRows + 2R rows, all in flight at once) and aligns each to the
+// cell grid with the next lane's word (DPP) and one v_alignbit. The AND of
+// 2R+1 rows, eroded horizontally by R cells with the neighbouring lanes'
+// words, is a row's validity in cell space; the output word, whose alignment
+// to the cell grid varies from row to row, is one more funnel shift of the
+// lane's and its left neighbour's cell words. Lanes 2..61 store output words;
+// lanes 0, 1, 62 and 63 only supply neighbours (lane 1's erosion needs lane
+// 0's cells, lane 61's needs lane 62's, whose alignment needs lane 63's word).
+// Words only partly owned by the region's row are updated with atomics on
+// the row's own bits; all reads stay inside the source buffer's rows.
+constexpr int kVsOut = 60;   // output words per wave and row
+constexpr int kVsRows = 16;  // rows per wave
 
-// Bits [o, o+64) of the 192-bit little-endian value (w0, w1, w2), o < 128.
-__device__ __forceinline__ uint64_t bits3(uint64_t w0, uint64_t w1, uint64_t w2, int o)
+__device__ __forceinline__ uint32_t lane_up(uint32_t v)  // v of lane+1 (lane 63: 0)
 {
-    if (o >= 64) {
-        o -= 64;
-        w0 = w1;
-        w1 = w2;
-    }
-    return o == 0 ? w0 : (w0 >> o) | (w1 << (64 - o));
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane_down(uint32_t v)  // v of lane-1 (lane 0: 0)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
 }
 
-// One thread per output mask word of one row of the region. For the
-// word's interior cells [lo, hi): each of the 2R+1 source rows is read
-// as the bits of cells [lo-R, hi+R) (at most three words, all loaded up
-// front), the rows are ANDed, and the result is eroded horizontally by
-// ANDing its 2R+1 shifts. Words only partly owned by the region's row
-// are updated with atomics on the row's own bits.
+// Bits [lo, hi) of a 32-bit word whose first bit is at 0, lo and hi clamped.
+__device__ __forceinline__ uint32_t bit_range(int32_t lo, int32_t hi)
+{
+    lo = max(lo, 0);
+    hi = min(hi, 32);
+    if (lo >= hi) return 0u;
+    const int n = hi - lo;
+    return (n == 32 ? ~0u : ((1u << n) - 1u)) << lo;
+}
+
+__device__ __forceinline__ int32_t clamp32(int64_t v, int32_t lo, int32_t hi)
+{
+    return (int32_t)min(max(v, (int64_t)lo), (int64_t)hi);
+}
+
+// Validity of rows [y0, y1) (y1 - y0 <= kVsRows) and output words
+// [wx*kVsOut, wx*kVsOut + kVsOut) of each of the region's rows. wx, y0 and
+// y1 are wave-uniform: every row and word offset below is scalar, advanced
+// by one stride per row, and lanes only add their own 32-bit word index.
+// Waves whose words lie inside the source rows and the output rows (all but
+// the first and last wave of a row, and strips at the buffer's top and
+// bottom) take unchecked loads and whole-word stores.
+template <int R>
+__device__ __forceinline__ void validity_strip(const Region &rg, const MaskView &src, int has_src,
+                                               const MaskView &dst, int64_t wx, int64_t y0, int64_t y1)
+{
+    constexpr int NL = kVsRows + 2 * R;  // source rows y0-R .. y0+kVsRows-1+R
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t xc = rg.ox + wx * (32 * kVsOut) - 64;  // lane 2's cells start at the wave's first word
+    const uint32_t *sbits = reinterpret_cast<const uint32_t *>(src.bits);
+    uint32_t *dbits = reinterpret_cast<uint32_t *>(dst.bits);
+
+    // Interior cells [R, W-R) of the lane's cell word; the same in every row.
+    uint32_t imask = ~0u;
+    if (xc < R || xc + 64 * 32 > rg.W - R) {
+        const int32_t u0 = clamp32(R - xc, -4096, 4096), u1 = clamp32(rg.W - R - xc, -4096, 4096);
+        imask = bit_range(u0 - 32 * lane, u1 - 32 * lane);
+    }
+
+    uint32_t A[NL];
+    if (has_src) {
+        uint32_t Wd[NL];
+        uint32_t sh[NL];
+        int64_t p = src.off + (y0 - R - src.y0) * src.stride + (xc - src.x0);  // bit of cell (xc, y0-R)
+        if (xc >= src.lx0 && xc + 64 * 32 <= src.lx1 && y0 - R >= src.ly0 && y0 + kVsRows + R <= src.ly1) {
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                const uint32_t *rowp = sbits + (p >> 5);
+                Wd[j] = rowp[lane];
+                sh[j] = (uint32_t)(p & 31);
+                p += src.stride;
+            }
+        } else {
+            // Rows the buffer does not hold read as zeros (cells outside the
+            // raster); lanes outside the row's own words read the nearest
+            // one and discard it, rows outside the buffer a word of its first
+            // row, so every load stays inside the buffer.
+            const int64_t safe = (src.off + (src.ly0 - src.y0) * src.stride + (src.lx0 - src.x0)) >> 5;
+            const int64_t dlo = src.lx0 - xc, dhi = src.lx1 - 1 - xc;  // row's cells relative to xc
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                const int64_t yy = y0 - R + j;
+                const int64_t base = p >> 5;  // word of lane 0
+                const int64_t lo = ((p + dlo) >> 5) - base, hi = ((p + dhi) >> 5) - base;
+                const bool ok = yy >= src.ly0 && yy < src.ly1 && lo <= 63 && hi >= 0;
+                const uint32_t *rowp = sbits + (ok ? base : safe);
+                const int32_t lo32 = ok ? clamp32(lo, 0, 63) : 0, hi32 = ok ? clamp32(hi, 0, 63) : 0;
+                const int32_t lc = min(max(lane, lo32), hi32);
+                const uint32_t w = rowp[lc];
+                Wd[j] = (ok && lc == lane) ? w : 0u;
+                sh[j] = (uint32_t)(p & 31);
+                p += src.stride;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j) A[j] = __builtin_amdgcn_alignbit(lane_up(Wd[j]), Wd[j], sh[j]);
+    }
+
+    // Output rows: lane l stores word wb + l of the row (lanes 2..61, up to
+    // the row's last word), whose cells start s = ob & 31 cells before the
+    // lane's own cell word. A wave past the first and before the last
+    // kVsOut + 2 words of every row never meets a partly owned word.
+    int64_t ob = dst.off + (y0 - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, y0)
+    const bool whole = wx > 0 && ((rg.w - 1) >> 5) >= wx * kVsOut + kVsOut;
+    const bool store_lane = lane >= 2 && lane < 2 + kVsOut;
+#pragma unroll
+    for (int i = 0; i < kVsRows; ++i) {
+        const int64_t y = y0 + i;
+        if (y >= y1) break;
+        const bool row_in = y >= R && y < rg.H - R;
+        uint32_t v;
+        if (has_src) {
+            uint32_t a = A[i];
+#pragma unroll
+            for (int j = 1; j <= 2 * R; ++j) a &= A[i + j];
+            const uint32_t lo = lane_down(a), hi = lane_up(a);
+            uint32_t e = a;
+#pragma unroll
+            for (int c = 1; c <= R; ++c) {
+                e &= __builtin_amdgcn_alignbit(hi, a, (uint32_t)c);        // cell + c
+                e &= __builtin_amdgcn_alignbit(a, lo, (uint32_t)(32 - c));  // cell - c
+            }
+            v = row_in ? e : 0u;
+        } else {
+            v = row_in ? ~0u : 0u;
+        }
+        v &= imask;
+        const uint32_t s = (uint32_t)(ob & 31);
+        const int64_t wb = (ob >> 5) + wx * kVsOut - 2;
+        uint32_t *rowd = dbits + wb;
+        const uint32_t prev = lane_down(v);
+        const uint32_t out = s == 0 ? v : __builtin_amdgcn_alignbit(v, prev, 32u - s);
+        if (whole) {
+            if (store_lane) rowd[lane] = out;
+        } else {
+            const int64_t oe = ob + rg.w;  // bit after the row
+            const int32_t last = clamp32(((oe - 1) >> 5) - wb, -1, 64);  // lane of the row's last word
+            const uint32_t e = (uint32_t)(oe & 31);
+            if (store_lane && lane <= last) {
+                uint32_t own = ~0u;
+                if (wx == 0 && lane == 2) own &= ~0u << s;             // the row's first word
+                if (e != 0 && lane == last) own &= (1u << e) - 1u;     // the row's last word
+                if (own == ~0u) {
+                    rowd[lane] = out;
+                } else {
+                    atomicAnd(rowd + lane, ~own | (out & own));
+                    atomicOr(rowd + lane, out & own);
+                }
+            }
+        }
+        ob += dst.stride;
+    }
+}
+
+// One wave per (word group, strip): wave t of the launch takes word group
+// t % waves_x of strip t / waves_x.
 template <int R>
 __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, int has_src, MaskView dst,
-                                                       int64_t row0)
+                                                       uint32_t waves_x)
 {
-    const int64_t y = row0 + blockIdx.y;
-    const int64_t ob = dst.off + (y - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, y)
-    const int64_t wfirst = ob >> 6, wlast = (ob + rg.w - 1) >> 6;
-    const int64_t wi = wfirst + (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (wi > wlast) return;
-
-    const int64_t wb = wi << 6;
-    const int64_t own_lo = max(wb, ob), own_hi = min(wb + 64, ob + rg.w);
-    const uint64_t own = (own_hi - own_lo == 64) ? ~0ull : (((1ull << (own_hi - own_lo)) - 1) << (own_lo - wb));
-
-    uint64_t val = 0;
-    if (y >= R && y < rg.H - R) {
-        const int64_t lo = max(own_lo, ob + (R - rg.ox));
-        const int64_t hi = min(own_hi, ob + (rg.W - R - rg.ox));
-        if (lo < hi) {
-            const int n = (int)(hi - lo);
-            uint64_t acc = (n == 64) ? ~0ull : ((1ull << n) - 1);
-            if (has_src) {
-                const int64_t xlo = rg.ox + (lo - ob);  // first interior cell of the word
-                uint64_t w[2 * R + 1][3];
-                int sh[2 * R + 1];
-#pragma unroll
-                for (int j = 0; j <= 2 * R; ++j) {
-                    const int64_t a = src.off + (y - R + j - src.y0) * src.stride + (xlo - R - src.x0);
-                    const int64_t base = a >> 6, last = (a + n + 2 * R - 1) >> 6;
-                    sh[j] = (int)(a & 63);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) w[j][k] = (base + k <= last) ? src.bits[base + k] : 0ull;
-                }
-                // Rows aligned to cell xlo-R: bits [0, 64) and [64, 64+2R).
-                uint64_t v0 = ~0ull, v1 = ~0ull;
-#pragma unroll
-                for (int j = 0; j <= 2 * R; ++j) {
-                    v0 &= bits3(w[j][0], w[j][1], w[j][2], sh[j]);
-                    if (R > 0) v1 &= bits3(w[j][0], w[j][1], w[j][2], sh[j] + 64);
-                }
-                uint64_t e = v0;
-#pragma unroll
-                for (int c = 1; c <= 2 * R; ++c) e &= (v0 >> c) | (v1 << (64 - c));
-                acc &= e;
-            }
-            val = acc << (lo - wb);
-        }
-    }
-    uint64_t *word = dst.bits + wi;
-    if (own == ~0ull) {
-        *word = val;
-    } else {
-        atomicAnd((unsigned long long *)word, (unsigned long long)(~own | val));
-        atomicOr((unsigned long long *)word, (unsigned long long)val);
-    }
+    const uint32_t wv = blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t wx = wv % waves_x, strip = wv / waves_x;
+    const int64_t y0 = rg.oy + (int64_t)strip * kVsRows;
+    const int64_t y1 = min(y0 + (int64_t)kVsRows, rg.oy + rg.h);
+    if (y0 >= y1) return;  // past the last strip
+    validity_strip<R>(rg, src, has_src, dst, wx, y0, y1);
 }
 
 // ---------------------------------------------------------------------------
@@ -665,6 +806,19 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
+
+// Launch-shape tuning (never changes a result).
+static Tuning g_tuning{64, {0, 0, 0, 0, 0, 0, 0}};
+static int sweep_rows() { return g_tuning.sweep_rows; }
+
+Tuning get_tuning() { return g_tuning; }
+
+bool set_tuning(const Tuning &t)
+{
+    if (t.sweep_rows < 4 || t.sweep_rows > 4096) return false;
+    g_tuning = t;
+    return true;
+}
 
 template <int OP>
 static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
@@ -680,15 +834,19 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
     bool aligned = al8(s.p) && (s.stride & 1) == 0 && ((rg.ox - s.sx) & 1) == 0;
     for (int o = 0; o < nout; ++o) aligned = aligned && al8(d[o].p) && (d[o].stride & 1) == 0;
     a.aligned = aligned ? 1 : 0;
+    a.rows = sweep_rows();
     const int64_t gx = cdiv(rg.w, kSweepCols);
-    for (int64_t yb = 0; yb < cdiv(rg.h, kSweepRows); yb += kMaxGridY) {
+    a.blocks_x = (uint32_t)gx;
+    // One 1-D launch per run of strips that keeps the block count in 32 bits.
+    const int64_t strips_per_launch = std::max<int64_t>(1, ((int64_t)1 << 30) / gx);
+    for (int64_t sb = 0; sb < cdiv(rg.h, a.rows); sb += strips_per_launch) {
         Region sub = rg;
-        sub.oy = rg.oy + yb * kSweepRows;
-        sub.h = min(rg.h - yb * kSweepRows, kMaxGridY * kSweepRows);
+        sub.oy = rg.oy + sb * a.rows;
+        sub.h = std::min<int64_t>(rg.h - sb * a.rows, strips_per_launch * a.rows);
         DstView e0 = d[0], e1 = d1;
-        e0.p += yb * kSweepRows * e0.stride;
-        e1.p += yb * kSweepRows * e1.stride;
-        dim3 grid((unsigned)gx, (unsigned)cdiv(sub.h, kSweepRows));
+        e0.p += sb * a.rows * e0.stride;
+        e1.p += sb * a.rows * e1.stride;
+        const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
         hipLaunchKernelGGL(horn_sweep<OP>, grid, dim3(kSweepThreads), 0, st, a, sub, s, e0, e1);
     }
     return hipGetLastError();
@@ -774,16 +932,20 @@ hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has
                            hipStream_t st)
 {
     if (rg.w <= 0 || rg.h <= 0) return hipSuccess;
-    // Words a row of the region can touch: at most w/64 + 2.
-    const int64_t words = rg.w / 64 + 2;
-    const int64_t gx = cdiv(words, 256);
-    for (int64_t yb = 0; yb < rg.h; yb += kMaxGridY) {
-        const int64_t rows = min(rg.h - yb, kMaxGridY);
-        dim3 grid((unsigned)gx, (unsigned)rows);
+    // Waves per strip: the row's words (at most w/32 + 2), kVsOut per wave.
+    const int64_t waves_x = cdiv(rg.w / 32 + 2, kVsOut);
+    const int64_t strips_per_launch = std::max<int64_t>(1, ((int64_t)1 << 30) / waves_x);
+    for (int64_t sb = 0; sb < cdiv(rg.h, kVsRows); sb += strips_per_launch) {
+        Region sub = rg;
+        sub.oy = rg.oy + sb * kVsRows;
+        sub.h = std::min<int64_t>(rg.h - sb * kVsRows, strips_per_launch * kVsRows);
+        const int64_t waves = waves_x * cdiv(sub.h, kVsRows);
+        const dim3 grid((unsigned)cdiv(waves, 4));
         switch (r) {
-#define STRATA_VALIDITY_CASE(RR)                                                                            \
-    case RR:                                                                                                \
-        hipLaunchKernelGGL(validity_kernel<RR>, grid, dim3(256), 0, st, rg, src, has_src, dst, rg.oy + yb); \
+#define STRATA_VALIDITY_CASE(RR)                                                                             \
+    case RR:                                                                                                 \
+        hipLaunchKernelGGL(validity_kernel<RR>, grid, dim3(256), 0, st, sub, src, has_src, dst,               \
+                           (uint32_t)waves_x);                                                               \
         break;
             STRATA_VALIDITY_CASE(0)
             STRATA_VALIDITY_CASE(1)
@@ -841,6 +1003,13 @@ __global__ void selfcheck_kernel(int kind, uint64_t lo, uint64_t n, unsigned lon
             if (x >= 0.0f && x <= kTanPi8) {
                 ran = true;
                 differ = f2u(atan32_small(x)) != f2u(atan32(x));
+            }
+        } else if (kind == 3) {
+            // The slope fast path on s = gx^2 + gy^2 against the full one.
+            const float x = u2f((uint32_t)(lo + i));
+            if (slope_small_ok(x)) {
+                ran = true;
+                differ = f2u(atan32_small(sqrt_fast(x))) != f2u(atan32(__builtin_sqrtf(x)));
             }
         } else {
             // |n| in [2^-30, 2^30) with random sign and mantissa, d in [1, 2^30).

@@ -36,12 +36,14 @@ struct DstView {
     int64_t stride;
 };
 
-// Validity bits: cell (x, y) is bit off + (y - y0)*stride + (x - x0).
+// Validity bits: cell (x, y) is bit off + (y - y0)*stride + (x - x0), held
+// for cells [lx0, lx1) x [ly0, ly1) (the buffer's extent in the raster).
 struct MaskView {
     uint64_t *bits;
     int64_t off;
     int64_t stride;
     int64_t x0, y0;
+    int64_t lx0, lx1, ly0, ly1;
 };
 
 // Resolved per-call constants: k[] as the C ABI's strata_kernel, taps
@@ -52,6 +54,14 @@ struct Params {
     float k[16];
     float taps[kMaxTaps];
 };
+
+// Launch-shape knobs (results never depend on them).
+struct Tuning {
+    int32_t sweep_rows;  // rows per block of the radius-1 sweep
+    int32_t reserved[7];
+};
+Tuning get_tuning();
+bool set_tuning(const Tuning &t);
 
 // Data of every cell in the region: the op over interior cells, NaN over
 // the edge ring. nout is 1 or 2 (Gradient). Returns a HIP error code.

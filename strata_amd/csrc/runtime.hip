@@ -213,7 +213,7 @@ SrcView src_view(const strata_raster &r, int64_t sx, int64_t sy, int64_t W, int6
     return v;
 }
 
-MaskView mask_view(const strata_raster &r, int64_t x0, int64_t y0)
+MaskView mask_view(const strata_raster &r, int64_t x0, int64_t y0, int64_t W, int64_t H)
 {
     MaskView m;
     m.bits = r.valid;
@@ -221,6 +221,10 @@ MaskView mask_view(const strata_raster &r, int64_t x0, int64_t y0)
     m.stride = r.stride;
     m.x0 = x0;
     m.y0 = y0;
+    m.lx0 = std::max<int64_t>(0, x0);
+    m.ly0 = std::max<int64_t>(0, y0);
+    m.lx1 = std::min<int64_t>(W, x0 + r.width);
+    m.ly1 = std::min<int64_t>(H, y0 + r.height);
     return m;
 }
 
@@ -403,9 +407,9 @@ int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st
     }
     for (int o = 0; o < op.nout; ++o) {
         if (!op.dst[o]->valid) continue;
-        const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o]);
+        const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o], op.W, op.H);
         const bool has_src = op.src->valid != nullptr;
-        const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy) : dm;
+        const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy, op.W, op.H) : dm;
         hipEvent_t v0 = timed ? timing_event(st) : nullptr;
         HIP_TRY(launch_validity(P.radius, rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");
         if (timed) {
@@ -485,7 +489,7 @@ const char *strata_version(void) { return "strata-hip 0.1 gfx950"; }
 
 int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, uint64_t *first, uint64_t *checked)
 {
-    if (kind < 0 || kind > 2) return fail(STRATA_EINVAL, "strata: unknown self-check %d", kind);
+    if (kind < 0 || kind > 3) return fail(STRATA_EINVAL, "strata: unknown self-check %d", kind);
     unsigned long long *d = nullptr;
     HIP_TRY(hipMalloc((void **)&d, 3 * sizeof(unsigned long long)), "self-check buffer");
     unsigned long long h[3] = {0, ~0ull, 0};
@@ -497,6 +501,25 @@ int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, ui
     if (mismatches) *mismatches = h[0];
     if (first) *first = h[1];
     if (checked) *checked = h[2];
+    g_err.clear();
+    return STRATA_OK;
+}
+
+int strata_get_tuning(strata_tuning *t)
+{
+    if (!t) return fail(STRATA_EINVAL, "strata: nil tuning");
+    const Tuning g = get_tuning();
+    memset(t, 0, sizeof *t);
+    t->sweep_rows = g.sweep_rows;
+    return STRATA_OK;
+}
+
+int strata_set_tuning(const strata_tuning *t)
+{
+    if (!t) return fail(STRATA_EINVAL, "strata: nil tuning");
+    Tuning g = get_tuning();
+    g.sweep_rows = t->sweep_rows;
+    if (!set_tuning(g)) return fail(STRATA_EINVAL, "strata: sweep_rows %d outside [4, 4096]", t->sweep_rows);
     g_err.clear();
     return STRATA_OK;
 }

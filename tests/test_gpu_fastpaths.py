@@ -32,6 +32,14 @@ def test_atan_small_exhaustive(gpu):
     assert bad == 0, f"small atan differs at bits {first:#x}"
 
 
+def test_slope_small_path_exhaustive(gpu):
+    """Every s = gx^2 + gy^2 the fused slope test accepts: +0 and
+    [2^-96, 0x3e2fb0cd], whose rounded roots are all <= tan(pi/8)."""
+    bad, first, checked = selfcheck(3, 0, 0x3E2FB0CD + 1)
+    assert checked == (0x3E2FB0CD - 0x0F800000 + 1) + 1
+    assert bad == 0, f"slope fast path differs at bits {first:#x}"
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_div_fast_random(gpu, seed):
     bad, first, checked = selfcheck(2, seed, 1 << 31)
