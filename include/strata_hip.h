@@ -262,6 +262,74 @@ int strata_timing_begin(void);
  * kernel durations and launch counts. */
 int strata_timing_end(double *data_ms, int64_t *data_launches, double *validity_ms, int64_t *validity_launches);
 
+/* ------------------------------------------------------------------ */
+/* Batches: many tiles, mixed ops, one persistent launch               */
+/* ------------------------------------------------------------------ */
+/*
+ * No single counterpart in the reference: a batch is many independent
+ * exec.ProcessN calls (internal/exec/process.go:28), one per task, each
+ * on its own rasters (its own edge ring, terrain/doc.go:89-90), followed
+ * by point ops applied to the task's output in place, as a chain
+ * Slope -> transfer.Lookup -> transfer.Reclass written as separate calls
+ * (transfer/example_test.go:90-129) or through exec.Pipeline
+ * (internal/exec/pipeline.go:11-187). Every output is bit-identical to
+ * those calls.
+ *
+ * Stencil ops a batch runs: Slope, Aspect, Hillshade, Curvature (radius
+ * 1, no fit) and focal Mean, Min, Max at radius 1..3. Point ops: Lookup
+ * and Reclass with at most 64 taps each. All rasters live in device
+ * memory; tasks must not write memory another task reads.
+ */
+typedef struct strata_task {
+    strata_kernel k;       /* the stencil op                              */
+    strata_kernel post[2]; /* point ops applied to its output, in order    */
+    int32_t npost;         /* 0..2                                         */
+    int32_t reserved;
+    strata_raster src;     /* the tile, e.g. a window of a resident raster */
+    strata_raster dst;     /* its output, the same width and height        */
+} strata_task;
+
+typedef struct strata_batch_options {
+    int32_t rows;          /* rows per work unit of radius-1 ops; 0: 64    */
+    int32_t blocks_per_cu; /* persistent workgroups per CU; 0: occupancy   */
+    int32_t graph;         /* 1: run as a captured HIP graph               */
+    int32_t reserved;
+    void *stream;          /* hipStream_t; NULL: a stream of the batch's own */
+} strata_batch_options;
+
+typedef struct strata_batch_info {
+    int64_t tasks;
+    int64_t cells;
+    int64_t data_units;
+    int64_t validity_units;
+    int32_t rows;
+    int32_t blocks;
+    int32_t graph;
+    int32_t reserved;
+} strata_batch_info;
+
+typedef struct strata_batch strata_batch;
+
+/* Checks the tasks (STRATA_EINVAL before anything runs) and builds the
+ * device work list. The caller keeps the rasters alive while the batch
+ * is used; kernel taps are copied. */
+int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch_options *opts,
+                        strata_batch **out);
+/* Enqueues the whole batch (one persistent launch, or its graph) on the
+ * batch's stream and returns. */
+int strata_batch_run(strata_batch *b);
+/* Enqueues the same tasks through the plain path: one strata_process call
+ * per task and per point op (the per-tile launch baseline). */
+int strata_batch_run_per_tile(strata_batch *b);
+/* Times the batch at each candidate launch shape (rows per unit, blocks
+ * per CU), reps runs each, and keeps the fastest; results never depend
+ * on the shape. The chosen shape is written to chosen when not NULL. */
+int strata_batch_tune(strata_batch *b, int reps, strata_batch_options *chosen);
+/* Waits for the batch's stream. */
+int strata_batch_sync(strata_batch *b);
+int strata_batch_info_get(const strata_batch *b, strata_batch_info *out);
+void strata_batch_destroy(strata_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

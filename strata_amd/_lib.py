@@ -130,6 +130,40 @@ class Sink(ctypes.Structure):
     ]
 
 
+class Task(ctypes.Structure):
+    _fields_ = [
+        ("k", Kernel),
+        ("post", Kernel * 2),
+        ("npost", c_int32),
+        ("reserved", c_int32),
+        ("src", Raster),
+        ("dst", Raster),
+    ]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("rows", c_int32),
+        ("blocks_per_cu", c_int32),
+        ("graph", c_int32),
+        ("reserved", c_int32),
+        ("stream", c_void_p),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("tasks", c_int64),
+        ("cells", c_int64),
+        ("data_units", c_int64),
+        ("validity_units", c_int64),
+        ("rows", c_int32),
+        ("blocks", c_int32),
+        ("graph", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
 # The symbols include/strata_hip.h declares, for the export check.
 EXPORTS = (
     "strata_init",
@@ -143,6 +177,13 @@ EXPORTS = (
     "strata_selfcheck",
     "strata_timing_begin",
     "strata_timing_end",
+    "strata_batch_create",
+    "strata_batch_run",
+    "strata_batch_run_per_tile",
+    "strata_batch_tune",
+    "strata_batch_sync",
+    "strata_batch_info_get",
+    "strata_batch_destroy",
 )
 
 _lib = None
@@ -191,15 +232,22 @@ def lib() -> ctypes.CDLL:
 
 
 def _bind_optional(L) -> None:
-    """Bind the batch and serving entry points when the build has them."""
+    """Bind the batch and serving entry points."""
     for name, argtypes, restype in _OPTIONAL:
-        fn = getattr(L, name, None)
-        if fn is not None:
-            fn.argtypes = argtypes
-            fn.restype = restype
+        fn = getattr(L, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
 
 
-_OPTIONAL: list = []
+_OPTIONAL: list = [
+    ("strata_batch_create", [POINTER(Task), c_int, POINTER(BatchOptions), POINTER(c_void_p)], c_int),
+    ("strata_batch_run", [c_void_p], c_int),
+    ("strata_batch_run_per_tile", [c_void_p], c_int),
+    ("strata_batch_tune", [c_void_p, c_int, POINTER(BatchOptions)], c_int),
+    ("strata_batch_sync", [c_void_p], c_int),
+    ("strata_batch_info_get", [c_void_p, POINTER(BatchInfo)], c_int),
+    ("strata_batch_destroy", [c_void_p], None),
+]
 
 
 class KernelTiming:

@@ -58,7 +58,7 @@ __device__ __forceinline__ float from_right(float v, float edge)
 // ---------------------------------------------------------------------------
 //
 // Layout: a block of 256 lanes covers 512 columns, two adjacent columns per
-// lane, and walks down a strip of kSweepRows rows. Each input row is loaded
+// lane, and walks down a strip of rows (the tuned strip height). Each input row is loaded
 // once per strip (a dwordx2 per lane when rows are 8-byte aligned); the cell
 // left of a lane's pair comes from lane-1 and the cell right of it from
 // lane+1 through DPP wave shifts, and each wave's two outer columns from one
@@ -71,7 +71,6 @@ __device__ __forceinline__ float from_right(float v, float edge)
 
 constexpr int kSweepThreads = 256;
 constexpr int kSweepCols = kSweepThreads * 2;
-constexpr int kSweepRows = 64;
 
 struct HornArgs {
     float k[8];
@@ -91,7 +90,7 @@ struct Row {
 // Correctly rounded sqrtf for x = +0 or x in [2^-96, FLT_MAX]: the hardware
 // root and the one-ulp correction the compiler's full sequence applies,
 // without its denormal scaling and special-value fix-up (not needed there).
-__device__ __forceinline__ bool sqrt_fast_ok(float x)
+__device__ __forceinline__ unsigned sqrt_fast_ok(float x)
 {
     const uint32_t u = f2u(x);  // +0, or [2^-96, FLT_MAX] (0x0f800000 .. 0x7f7fffff)
     return (u == 0u) | ((u - 0x0f800000u) <= (0x7f7fffffu - 0x0f800000u));
@@ -109,7 +108,7 @@ __device__ __forceinline__ float sqrt_fast(float x)
 // compiler's refinement sequence (reciprocal refined by one Newton step, two
 // residual corrections) without the range scaling and fix-up, which never
 // apply there.
-__device__ __forceinline__ bool div_fast_ok(float n, float d)
+__device__ __forceinline__ unsigned div_fast_ok(float n, float d)
 {
     const float an = u2f(f2u(n) & 0x7fffffffu);
     return (an >= 0x1p-30f) & (an <= 0x1p30f) & (d >= 1.0f) & (d <= 0x1p30f);
@@ -138,14 +137,16 @@ __device__ __forceinline__ float atan32_small(float t)
 }
 
 // True on every active lane (a wave-uniform vote without exec branching).
-__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+// The tests it combines are 0/1 values joined with &, which keeps them
+// free of short-circuit branches.
+__device__ __forceinline__ bool wave_all(unsigned p) { return __builtin_amdgcn_ballot_w64(p == 0u) == 0; }
 
 // s = gx^2 + gy^2 whose rounded root m satisfies both fast paths: s = +0 or
 // s in [2^-96, S*], where S* (0x3e2fb0cd) is the largest float whose
 // correctly rounded root is <= tan(pi/8) as float32. Then m = sqrt_fast(s)
 // and Atan32(m) = atan32_small(m).
 constexpr uint32_t kSlopeSmallMax = 0x3e2fb0cdu;
-__device__ __forceinline__ bool slope_small_ok(float s)
+__device__ __forceinline__ unsigned slope_small_ok(float s)
 {
     const uint32_t u = f2u(s);
     return (u == 0u) | ((u - 0x0f800000u) <= (kSlopeSmallMax - 0x0f800000u));
@@ -246,11 +247,18 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
     }
 }
 
+// The identity on a sweep's output values.
+struct NoPost {
+    __device__ float operator()(float v) const { return v; }
+};
+
 // PAIRS: every pair of the block lies inside the buffer and the region, at
-// 8-byte-aligned addresses, so rows move as dwordx2 loads and stores.
-template <int OP, bool PAIRS>
+// 8-byte-aligned addresses, so rows move as dwordx2 loads and stores. `post`
+// maps every stored value (edge NaNs included) of the first output.
+template <int OP, bool PAIRS, class Post = NoPost>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
-                                      const DstView &d1, int64_t xb, int64_t y0, int64_t y1)
+                                      const DstView &d1, int64_t xb, int64_t y0, int64_t y1,
+                                      const Post &post = Post())
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
     const int lane = threadIdx.x & 63;
@@ -322,7 +330,7 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                 horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
                 const bool ri = yy >= 1 && yy < rg.H - 1;
                 const float nan = edge_nan();
-                store(d0, yy, (ri && ci0) ? o00 : nan, (ri && ci1) ? o01 : nan);
+                store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
                 if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
             }
         }
@@ -801,6 +809,145 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
 }
 
 // ---------------------------------------------------------------------------
+// batch_kernel: many tiles and mixed ops in one persistent launch
+// ---------------------------------------------------------------------------
+//
+// Workgroups claim units from a counter in order (data units of every task,
+// then validity units) until the list is empty, so uneven tiles balance
+// across the chip and one launch (capturable in a graph) runs the whole
+// batch. The task of a unit is found by binary search over the tasks' first
+// units; the unit index is read back through readfirstlane so the search and
+// the task's fields are scalar.
+
+// The point ops of a batch op, applied to a stencil op's output values.
+struct ChainPost {
+    const BatchOp *op;
+    __device__ float operator()(float v) const
+    {
+        for (int i = 0; i < op->npost; ++i) {
+            const float *t = op->post_taps[i];
+            if (op->post_op[i] == STRATA_OP_LOOKUP) {
+                const int n = op->post_n[i] / 2;
+                v = lookup_of(v, t, t + n, n);
+            } else {
+                const int nb = (op->post_n[i] - 1) / 2;
+                v = reclass_of(v, t, nb, t + nb);
+            }
+        }
+        return v;
+    }
+};
+
+constexpr int kBatchMaxFocalR = 3;
+constexpr int kBatchLdsFloats = focal_lds_floats<FK_MEAN, kBatchMaxFocalR>();
+
+template <int OP>
+__device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t, int64_t lu)
+{
+    HornArgs a;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a.k[i] = op.k[i];
+    a.flags = op.flags;
+    a.aligned = t.aligned;
+    a.rows = (int32_t)t.rows;
+    a.blocks_x = t.blocks_x;
+    const int64_t xb = t.rg.ox + (lu % t.blocks_x) * kSweepCols;
+    const int64_t y0 = t.rg.oy + (lu / t.blocks_x) * t.rows;
+    const int64_t y1 = min(y0 + (int64_t)t.rows, t.rg.oy + t.rg.h);
+    const bool pairs = t.aligned && xb >= t.s.lx0 && xb + kSweepCols <= t.s.lx1 && xb >= t.rg.ox &&
+                       xb + kSweepCols <= t.rg.ox + t.rg.w;
+    const ChainPost post{&op};
+    if (pairs)
+        sweep<OP, true>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, post);
+    else
+        sweep<OP, false>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, post);
+}
+
+template <int R>
+__device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &t, int64_t lu, float *lds)
+{
+    FocalArgs a;
+    a.op = op.op;
+    a.flags = op.flags;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a.k[i] = op.k[i];
+    const PtrTaps taps{nullptr};
+    const int64_t tx0 = t.rg.ox + (lu % t.blocks_x) * kTileX;
+    const int64_t ty0 = t.rg.oy + (lu / t.blocks_x) * kTileY;
+    switch (op.kind) {
+    case BK_MEAN: focal_block<FK_MEAN, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
+    case BK_MIN: focal_block<FK_MIN, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
+    default: focal_block<FK_MAX, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
+    }
+}
+
+// The last task whose first unit (u0 or v0) is <= u.
+template <bool VALIDITY>
+__device__ __forceinline__ int64_t batch_find(const BatchTask *tasks, int64_t n, int64_t u)
+{
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        const int64_t first = VALIDITY ? tasks[mid].v0 : tasks[mid].u0;
+        if (first <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_clear(unsigned *counter) { *counter = 0u; }
+
+__global__ __launch_bounds__(256) void batch_kernel(const BatchHeader *hdr, const BatchTask *tasks,
+                                                    const BatchOp *ops, unsigned *counter)
+{
+    __shared__ float lds[kBatchLdsFloats];
+    __shared__ unsigned claimed;
+    const int64_t ntasks = hdr->ntasks, ndata = hdr->ndata, nval = hdr->nval;
+    for (;;) {
+        __syncthreads();  // every thread has read the previous claim
+        if (threadIdx.x == 0) claimed = atomicAdd(counter, 1u);
+        __syncthreads();
+        const int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+        if (u >= ndata + nval) return;
+        if (u < ndata) {
+            const BatchTask &t = tasks[batch_find<false>(tasks, ntasks, u)];
+            const BatchOp &op = ops[t.op];
+            const int64_t lu = u - t.u0;
+            if (op.kind == BK_HORN) {
+                switch (op.op) {
+                case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu); break;
+                case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu); break;
+                case STRATA_OP_HILLSHADE: batch_horn<STRATA_OP_HILLSHADE>(op, t, lu); break;
+                default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu); break;
+                }
+            } else {
+                switch (op.radius) {
+                case 1: batch_focal<1>(op, t, lu, lds); break;
+                case 2: batch_focal<2>(op, t, lu, lds); break;
+                default: batch_focal<3>(op, t, lu, lds); break;
+                }
+            }
+        } else {
+            const int64_t vu = u - ndata;
+            const BatchTask &t = tasks[batch_find<true>(tasks, ntasks, vu)];
+            const int64_t w = (vu - t.v0) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            if (w < t.vunits) {
+                const int64_t wx = w % t.vwaves_x, strip = w / t.vwaves_x;
+                const int64_t y0 = t.rg.oy + strip * kVsRows;
+                const int64_t y1 = min(y0 + (int64_t)kVsRows, t.rg.oy + t.rg.h);
+                switch (ops[t.op].radius) {
+                case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+                case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+                default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
@@ -962,6 +1109,33 @@ hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has
         }
     }
     return hipGetLastError();
+}
+
+void batch_units(const BatchOp &op, BatchTask &t, int rows)
+{
+    if (op.kind == BK_HORN) {
+        t.rows = (uint32_t)rows;
+        t.blocks_x = (uint32_t)cdiv(t.rg.w, kSweepCols);
+    } else {
+        t.rows = kTileY;
+        t.blocks_x = (uint32_t)cdiv(t.rg.w, kTileX);
+    }
+    t.vwaves_x = (uint32_t)cdiv(t.rg.w / 32 + 2, kVsOut);
+}
+
+hipError_t launch_batch(const BatchHeader *hdr, const BatchTask *tasks, const BatchOp *ops, unsigned *counter,
+                        int blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_clear, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(batch_kernel, dim3((unsigned)blocks), dim3(256), 0, st, hdr, tasks, ops, counter);
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 256, 0) != hipSuccess || n < 1) n = 1;
+    return n;
 }
 
 } // namespace strata

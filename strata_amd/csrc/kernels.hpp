@@ -77,4 +77,67 @@ hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has
 // selfcheck_kernel); dev3 holds {mismatches, first mismatch index, checked}.
 hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long long *dev3, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Batches: many tiles, mixed ops, one persistent launch over a work list
+// ---------------------------------------------------------------------------
+
+constexpr int kBatchPostTaps = 64;  // floats per point-op table in a batch op
+
+// Families of stencil work a batch runs.
+enum BatchKind : int32_t {
+    BK_HORN = 0,  // radius-1 terrain ops (horn_sweep's work)
+    BK_MEAN = 1,  // focal Mean, Min, Max at radius 1..3
+    BK_MIN = 2,
+    BK_MAX = 3,
+};
+
+// One op of a batch: the stencil op and the point ops (Lookup, Reclass)
+// applied to its output in order, which is the standalone chain of ops
+// cell for cell because point ops read only their own cell.
+struct BatchOp {
+    int32_t op, radius, flags, kind;
+    float k[16];
+    int32_t npost, pad;
+    int32_t post_op[2];
+    int32_t post_n[2];
+    float post_taps[2][kBatchPostTaps];
+};
+
+// One tile: an op over its own raster (its edge ring is its own), written to
+// its own output. Data work is split into units of one 512-column strip of
+// `rows` rows (radius-1 ops) or one focal tile; validity work into units of
+// four validity waves.
+struct BatchTask {
+    Region rg;
+    SrcView s;
+    DstView d;
+    MaskView sm, dm;
+    int32_t op;       // index into the batch's op table
+    int32_t has_src;  // the source has a mask
+    int32_t vunits;   // validity waves (0 when the output has no mask)
+    int32_t aligned;  // pair (8-byte) accesses allowed
+    uint32_t blocks_x;  // data units across
+    uint32_t rows;      // rows per data unit (radius-1 ops)
+    uint32_t vwaves_x;  // validity waves across
+    uint32_t pad;
+    int64_t u0;  // first data unit
+    int64_t v0;  // first validity unit (four waves each)
+};
+
+struct BatchHeader {
+    int64_t ntasks, ndata, nval;
+};
+
+// Units of each kind a task needs (data units and validity units of four
+// waves); rows is the strip height of radius-1 ops.
+void batch_units(const BatchOp &op, BatchTask &t, int rows);
+
+// Clears the claim counter (a kernel, so it can live in a captured graph)
+// and runs the persistent batch kernel with `blocks` workgroups.
+hipError_t launch_batch(const BatchHeader *hdr, const BatchTask *tasks, const BatchOp *ops, unsigned *counter,
+                        int blocks, hipStream_t st);
+
+// Workgroups of the batch kernel resident per CU (occupancy), for sizing.
+int batch_blocks_per_cu();
+
 } // namespace strata

@@ -20,12 +20,13 @@
 #include <string>
 #include <vector>
 
+#include "host.hpp"
 #include "kernels.hpp"
 #include "strata_hip.h"
 
 using namespace strata;
 
-namespace {
+namespace strata {
 
 thread_local std::string g_err;
 
@@ -44,12 +45,6 @@ int hip_fail(hipError_t e, const char *what)
 {
     return fail(STRATA_EDEVICE, "strata: %s: %s", what, hipGetErrorString(e));
 }
-
-#define HIP_TRY(expr, what)                                                                                  \
-    do {                                                                                                     \
-        hipError_t e_ = (expr);                                                                              \
-        if (e_ != hipSuccess) return hip_fail(e_, what);                                                     \
-    } while (0)
 
 constexpr int64_t kBytesPerCell = 4;
 constexpr int64_t kBandCells = 1 << 16;  // internal/exec/tile.go:27
@@ -150,6 +145,10 @@ int check_raster(const char *name, int i, const strata_raster &r)
         return fail(STRATA_EINVAL, "engine: %s[%d]: unknown memory kind %d", name, i, r.mem);
     return STRATA_OK;
 }
+
+} // namespace strata
+
+namespace {
 
 // A host-memory raster mirrored in device memory for the call. Device
 // memory holds the raster's whole span (cells and padding) and the mask
