@@ -11,8 +11,9 @@ over the timed steps, inputs already resident in HBM.
 Also reported: the roofline of the dominant kernel (the Slope data
 kernel) from HIP events the library records around each of its launches
 on the launch stream, the rest of the terrain suite (Aspect, Hillshade,
-profile/plan/mean Curvature, Gradient) on the same DEM, and the CPU
-baseline: the oracle (a scalar C restatement of the reference's kernels)
+profile/plan/mean Curvature, Gradient) on the same DEM, a batch of 4096
+uneven mixed-op tiles run as one persistent graph-captured launch against
+one launch set per tile, and the CPU baseline: the oracle (a scalar C restatement of the reference's kernels)
 timed on the host cores over a bounded sample of the same rows, whose
 output is also compared bit for bit with the GPU's.
 
@@ -51,6 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work (thread-seconds) to aim for")
     ap.add_argument("--sweep-rows", type=int, default=0,
                     help="rows per block of the radius-1 kernels (0: the library's default)")
+    ap.add_argument("--batch-tiles", type=int, default=4096,
+                    help="tiles in the batched-launch comparison (0: skip it)")
+    ap.add_argument("--batch-reps", type=int, default=10, help="timed runs of the batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     return ap.parse_args()
@@ -165,6 +169,12 @@ def main() -> int:
             log(f"bench: {name} {suite[name]}")
         del out2
 
+    batch = None
+    if args.batch_tiles > 0:
+        del out, sink
+        torch.cuda.empty_cache()
+        batch = batch_bench(args, dem, N, dev, log)
+
     if args.calibrate_copy:
         tmp = torch.empty_like(dem_data)
         tmp.copy_(dem_data)
@@ -199,9 +209,100 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity_sample": parity,
         "suite": suite,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def batch_tiles(n, N, seed):
+    """n tiles with sides log-uniform in [256, 2048], at uniform offsets
+    inside the N×N DEM, each with an op drawn uniformly from {Slope,
+    Hillshade, mean Curvature, focal Mean r=3}."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(256), np.log(2048)
+    w = np.minimum(np.exp(rng.uniform(lo, hi, n)).astype(np.int64), N)
+    h = np.minimum(np.exp(rng.uniform(lo, hi, n)).astype(np.int64), N)
+    x = (rng.random(n) * (N - w + 1)).astype(np.int64)
+    y = (rng.random(n) * (N - h + 1)).astype(np.int64)
+    op = rng.integers(0, 4, n)
+    return w, h, x, y, op
+
+
+def batch_bench(args, dem, N, dev, log):
+    """Many uneven tiles with mixed ops: one persistent, graph-captured,
+    tuned launch against one plain-path launch set per tile. Both write the
+    same output buffers; their outputs are compared bit for bit."""
+    import torch
+
+    from strata_amd import focal, terrain
+    from strata_amd.batch import Batch, Task
+    from strata_amd.raster import Float32Raster
+
+    ops = [terrain.slopeOp(terrain.SlopeOptions(CellSize=30)),
+           terrain.hillshadeOp(terrain.HillshadeOptions(CellSize=30, Azimuth=315, Altitude=45, ZFactor=1)),
+           terrain.curvatureOp(terrain.CurvatureOptions(CellSize=30, Type=terrain.CurvatureMean)),
+           focal.newBox(focal.BoxOptions(3), "mean")]
+    w, h, x, y, op = batch_tiles(args.batch_tiles, N, seed=4)
+    cells = int((w * h).sum())
+    out = torch.empty(cells, dtype=torch.float32, device=dev)
+    words = (cells + 63) // 64 + 1
+    mask = torch.zeros(words, dtype=torch.int64, device=dev)
+    tasks = []
+    off = 0
+    for i in range(len(w)):
+        wi, hi_, xi, yi = int(w[i]), int(h[i]), int(x[i]), int(y[i])
+        start = yi * N + xi
+        src = Float32Raster(dem.Data[start:start + (hi_ - 1) * N + wi], wi, hi_, N, dem.Valid, start)
+        dst = Float32Raster(out[off:off + wi * hi_], wi, hi_, wi, mask, off)
+        tasks.append(Task(ops[int(op[i])], src, dst))
+        off += wi * hi_
+    torch.cuda.synchronize()
+
+    b = Batch(tasks, graph=True)
+    shape = b.tune(reps=2)
+
+    def timed(fn, reps):
+        fn()
+        b.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        b.sync()
+        return (time.perf_counter() - t) / reps
+
+    per_tile_s = timed(b.run_per_tile, max(1, args.batch_reps // 5))
+    ref = out.clone()
+    ref_mask = mask.clone()
+    batch_s = timed(b.run, args.batch_reps)
+    identical = bool(torch.equal(out.view(torch.int32), ref.view(torch.int32)) and torch.equal(mask, ref_mask))
+    info = b.info()
+    b.close()
+    del ref, ref_mask
+    n = len(tasks)
+    res = {
+        "workload": f"{n} tiles, sides log-uniform 256-2048, ops uniform over Slope, Hillshade, mean Curvature, "
+                    f"focal Mean r=3, windows of the {N}x{N} DEM, each with its own output and mask",
+        "tiles": n, "cells": cells,
+        "batched": {"ms": round(batch_s * 1e3, 3), "tiles_per_s": round(n / batch_s, 1),
+                    "gcells_s": round(cells / batch_s / 1e9, 2),
+                    "hbm_frac": round(8.25 * cells / batch_s / 1e9 / HBM_PEAK_GBS, 4),
+                    "launch": "one persistent kernel in a captured HIP graph", "shape": shape,
+                    "work_units": info["data_units"] + info["validity_units"]},
+        "per_tile": {"ms": round(per_tile_s * 1e3, 3), "tiles_per_s": round(n / per_tile_s, 1),
+                     "gcells_s": round(cells / per_tile_s / 1e9, 2),
+                     "launch": "strata_process per tile (data and validity kernels)"},
+        "speedup": round(per_tile_s / batch_s, 2),
+        "identical": identical,
+    }
+    log(f"bench: batch {n} tiles {cells / 1e9:.2f} Gcells: batched {batch_s * 1e3:.2f} ms "
+        f"({res['batched']['gcells_s']} Gcells/s, shape {shape}), per tile {per_tile_s * 1e3:.2f} ms, "
+        f"speedup {res['speedup']}x, identical {identical}")
+    del out, mask
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(args, dem, out, slope, N, log):

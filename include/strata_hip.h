@@ -330,6 +330,77 @@ int strata_batch_sync(strata_batch *b);
 int strata_batch_info_get(const strata_batch *b, strata_batch_info *out);
 void strata_batch_destroy(strata_batch *b);
 
+/* ------------------------------------------------------------------ */
+/* Serving: window requests against a device-resident raster          */
+/* ------------------------------------------------------------------ */
+/*
+ * No counterpart in the reference (a library, not a server). A request is
+ * a window (x, y, w, h) of the resident raster; its result is what the
+ * chain gives on that window as a raster of its own: the stencil op
+ * (exec.ProcessN, internal/exec/process.go:28, NaN ring at the window's
+ * edge, terrain/doc.go:89-90) and then the point ops, bit for bit the same
+ * whether the request runs alone or among others.
+ *
+ * A scheduler thread admits pending requests in arrival order into
+ * batches (one persistent launch each, the batch kernel) as soon as a
+ * batch slot is free and their result memory fits the HBM budget; up to
+ * `slots` batches run at once on their own streams, so requests that
+ * arrive while others run join the next batch (continuous batching).
+ * Latency runs from a request's arrival to the moment its batch is seen
+ * complete.
+ */
+typedef struct strata_serve_config {
+    strata_raster src;     /* device-resident raster requests read        */
+    strata_kernel k;       /* the stencil op (as in a batch)              */
+    strata_kernel post[2]; /* point ops applied after it, in order        */
+    int32_t npost;
+    int32_t slots;         /* batches in flight at once; 0: 4             */
+    int64_t hbm_budget;    /* bytes of result memory; 0: 4 GiB            */
+    int32_t max_batch;     /* requests per batch; 0: 256                  */
+    int32_t rows;          /* rows per work unit of radius-1 ops; 0: 64   */
+    int32_t auto_release;  /* 1: drop each result when it completes       */
+    int32_t reserved;
+} strata_serve_config;
+
+typedef struct strata_window {
+    int64_t x, y, w, h;
+} strata_window;
+
+typedef struct strata_serve_stats {
+    int64_t requests;      /* completed                                   */
+    int64_t batches;
+    int64_t cells;
+    int64_t max_in_flight; /* requests admitted and not yet complete      */
+    int64_t admission_waits; /* times the queue head waited for memory    */
+    int64_t peak_bytes;    /* peak result memory in use                   */
+    double busy_ms;        /* wall time with at least one batch in flight */
+    double reserved;
+} strata_serve_stats;
+
+typedef struct strata_server strata_server;
+
+/* Checks the configuration (STRATA_EINVAL), allocates the result arena
+ * and starts the scheduler. */
+int strata_serve_create(const strata_serve_config *cfg, strata_server **out);
+/* Queues a request and returns at once with its id. A window outside the
+ * raster, or one whose result can never fit the budget, is STRATA_EINVAL. */
+int strata_serve_submit(strata_server *s, const strata_window *w, int64_t *id);
+/* Waits for request id; its latency in milliseconds when not NULL. */
+int strata_serve_wait(strata_server *s, int64_t id, double *latency_ms);
+/* Copies a completed, unreleased result out: w*h cells row by row and the
+ * validity of cell i at bit i of mask (ceil(w*h/64) words), both host or
+ * device memory; either may be NULL. */
+int strata_serve_fetch(strata_server *s, int64_t id, float *data, uint64_t *mask);
+/* Frees a completed result's memory. */
+int strata_serve_release(strata_server *s, int64_t id);
+/* Open-loop replay: submits windows[i] at arrival_s[i] seconds after the
+ * call starts (non-decreasing), waits for all, and writes each latency
+ * (from its scheduled arrival) to latency_ms[i]. */
+int strata_serve_replay(strata_server *s, const strata_window *windows, const double *arrival_s, int n,
+                        double *latency_ms);
+int strata_serve_stats_get(strata_server *s, strata_serve_stats *out);
+void strata_serve_destroy(strata_server *s);
+
 #ifdef __cplusplus
 }
 #endif

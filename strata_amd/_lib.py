@@ -164,6 +164,38 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("src", Raster),
+        ("k", Kernel),
+        ("post", Kernel * 2),
+        ("npost", c_int32),
+        ("slots", c_int32),
+        ("hbm_budget", c_int64),
+        ("max_batch", c_int32),
+        ("rows", c_int32),
+        ("auto_release", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
+class Window(ctypes.Structure):
+    _fields_ = [("x", c_int64), ("y", c_int64), ("w", c_int64), ("h", c_int64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("requests", c_int64),
+        ("batches", c_int64),
+        ("cells", c_int64),
+        ("max_in_flight", c_int64),
+        ("admission_waits", c_int64),
+        ("peak_bytes", c_int64),
+        ("busy_ms", ctypes.c_double),
+        ("reserved", ctypes.c_double),
+    ]
+
+
 # The symbols include/strata_hip.h declares, for the export check.
 EXPORTS = (
     "strata_init",
@@ -184,6 +216,14 @@ EXPORTS = (
     "strata_batch_sync",
     "strata_batch_info_get",
     "strata_batch_destroy",
+    "strata_serve_create",
+    "strata_serve_submit",
+    "strata_serve_wait",
+    "strata_serve_fetch",
+    "strata_serve_release",
+    "strata_serve_replay",
+    "strata_serve_stats_get",
+    "strata_serve_destroy",
 )
 
 _lib = None
@@ -247,6 +287,15 @@ _OPTIONAL: list = [
     ("strata_batch_sync", [c_void_p], c_int),
     ("strata_batch_info_get", [c_void_p, POINTER(BatchInfo)], c_int),
     ("strata_batch_destroy", [c_void_p], None),
+    ("strata_serve_create", [POINTER(ServeConfig), POINTER(c_void_p)], c_int),
+    ("strata_serve_submit", [c_void_p, POINTER(Window), POINTER(c_int64)], c_int),
+    ("strata_serve_wait", [c_void_p, c_int64, POINTER(ctypes.c_double)], c_int),
+    ("strata_serve_fetch", [c_void_p, c_int64, c_void_p, c_void_p], c_int),
+    ("strata_serve_release", [c_void_p, c_int64], c_int),
+    ("strata_serve_replay", [c_void_p, POINTER(Window), POINTER(ctypes.c_double), c_int,
+                             POINTER(ctypes.c_double)], c_int),
+    ("strata_serve_stats_get", [c_void_p, POINTER(ServeStats)], c_int),
+    ("strata_serve_destroy", [c_void_p], None),
 ]
 
 

@@ -25,12 +25,15 @@ struct strata_batch {
     std::vector<BatchOp> ops;                 // distinct ops
     std::vector<int32_t> op_of;               // op index of each task
     std::vector<int32_t> order;               // tasks in work-list order (largest first)
-    std::vector<BatchTask> list;              // the device work list's host copy
-    BatchHeader hdr{};
-    char *dev = nullptr;  // header | ops | tasks | counter
+    std::vector<BatchTask> list;              // tasks in work-list order (made by make_task)
+    std::vector<char> image;                  // host image of the device work lists
+    ListLayout layout{};
+    char *dev = nullptr;  // the work lists (see ListLayout)
     size_t dev_bytes = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side[2] = {nullptr, nullptr};  // the focal and validity lists' streams
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int rows = 64, blocks = 0, blocks_per_cu = 0, cus = 1;
     bool graph = false;
     hipGraph_t g = nullptr;
@@ -42,11 +45,15 @@ struct strata_batch {
         if (gx) (void)hipGraphExecDestroy(gx);
         if (g) (void)hipGraphDestroy(g);
         if (dev) (void)hipFree(dev);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t st : side)
+            if (st) (void)hipStreamDestroy(st);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
 
-namespace {
+namespace strata {
 
 int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -150,65 +157,122 @@ BatchTask make_task(const strata_task &t, int32_t op)
     return b;
 }
 
-// Lays the work list out for `rows` rows per radius-1 unit and uploads it.
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+size_t list_bytes(int nops, size_t ntasks)
+{
+    return 256 + align256(sizeof(BatchOp) * (size_t)nops) + 3 * align256(sizeof(BatchTask) * ntasks) + 256 * kBatchFamilies;
+}
+
+ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, const bool *masked, size_t n,
+                       int rows, char *out)
+{
+    ListLayout L;
+    L.ops = 256;
+    size_t count[kBatchFamilies] = {0, 0, 0};
+    for (size_t i = 0; i < n; ++i) {
+        count[ops[tasks[i].op].kind == BK_HORN ? BF_HORN : BF_FOCAL] += 1;
+        if (masked[i]) count[BF_VALIDITY] += 1;
+    }
+    size_t p = L.ops + align256(sizeof(BatchOp) * (size_t)nops);
+    for (int f = 0; f < kBatchFamilies; ++f) {
+        L.tasks[f] = p;
+        p += align256(sizeof(BatchTask) * count[f]);
+    }
+    L.counters = p;
+    L.bytes = p + 256 * kBatchFamilies;  // one 256-byte line per counter
+    memcpy(out + L.ops, ops, sizeof(BatchOp) * (size_t)nops);
+    size_t k[kBatchFamilies] = {0, 0, 0};
+    int64_t u[kBatchFamilies] = {0, 0, 0};
+    BatchTask *lt[kBatchFamilies];
+    for (int f = 0; f < kBatchFamilies; ++f) lt[f] = reinterpret_cast<BatchTask *>(out + L.tasks[f]);
+    for (size_t i = 0; i < n; ++i) {
+        BatchTask t = tasks[i];
+        batch_units(ops[t.op], t, rows);
+        const int64_t data = (int64_t)t.blocks_x * cdiv64(t.rg.h, t.rows);
+        t.vunits = masked[i] ? (int32_t)((int64_t)t.vwaves_x * cdiv64(t.rg.h, 16)) : 0;
+        const int f = ops[t.op].kind == BK_HORN ? BF_HORN : BF_FOCAL;
+        t.u0 = u[f];
+        t.u1 = u[f] += data;
+        lt[f][k[f]++] = t;
+        if (masked[i]) {
+            t.u0 = u[BF_VALIDITY];
+            t.u1 = u[BF_VALIDITY] += t.vunits;
+            lt[BF_VALIDITY][k[BF_VALIDITY]++] = t;
+        }
+    }
+    for (int f = 0; f < kBatchFamilies; ++f) {
+        L.h[f].ntasks = (int64_t)count[f];
+        L.h[f].nunits = u[f];
+    }
+    memcpy(out, L.h, sizeof L.h);
+    memset(out + L.counters, 0, 256 * kBatchFamilies);
+    return L;
+}
+
+BatchLaunch launch_view(const ListLayout &L, char *dev, const int *max_blocks)
+{
+    BatchLaunch b;
+    b.ops = reinterpret_cast<const BatchOp *>(dev + L.ops);
+    for (int f = 0; f < kBatchFamilies; ++f) {
+        b.hdr[f] = reinterpret_cast<const BatchHeader *>(dev) + f;
+        b.tasks[f] = reinterpret_cast<const BatchTask *>(dev + L.tasks[f]);
+        b.nunits[f] = L.h[f].nunits;
+        b.blocks[f] = (int)std::max<int64_t>(1, std::min<int64_t>(max_blocks[f], L.h[f].nunits));
+    }
+    b.counters = reinterpret_cast<unsigned *>(dev + L.counters);
+    return b;
+}
+
+} // namespace strata
+
+namespace {
+
+// Lays the work lists out for `rows` rows per radius-1 unit and uploads them.
 int build_list(strata_batch *b, int rows)
 {
     b->rows = rows;
-    int64_t u = 0, v = 0;
-    for (size_t n = 0; n < b->order.size(); ++n) {
-        const int i = b->order[n];
-        BatchTask &t = b->list[n];
-        const BatchOp &op = b->ops[b->op_of[i]];
-        batch_units(op, t, rows);
-        t.u0 = u;
-        u += (int64_t)t.blocks_x * cdiv64(t.rg.h, t.rows);
-        t.vunits = b->tasks[i].dst.valid ? (int32_t)((int64_t)t.vwaves_x * cdiv64(t.rg.h, 16)) : 0;
-        t.v0 = v;
-        v += cdiv64(t.vunits, 4);
-    }
-    if (u + v >= ((int64_t)1 << 31))
-        return fail(STRATA_EINVAL, "strata: batch of %lld work units exceeds the limit of 2^31",
-                    (long long)(u + v));
-    b->hdr.ntasks = (int64_t)b->list.size();
-    b->hdr.ndata = u;
-    b->hdr.nval = v;
-    const size_t oh = 0, oo = 256, ot = oo + ((sizeof(BatchOp) * b->ops.size() + 255) & ~(size_t)255),
-                 oc = ot + ((sizeof(BatchTask) * b->list.size() + 255) & ~(size_t)255);
-    const size_t bytes = oc + 256;
+    std::vector<char> masked(b->list.size());
+    for (size_t n = 0; n < b->list.size(); ++n) masked[n] = b->tasks[(size_t)b->order[n]].dst.valid != nullptr;
+    b->image.assign(list_bytes((int)b->ops.size(), b->list.size()), 0);
+    b->layout = write_lists(b->ops.data(), (int)b->ops.size(), b->list.data(),
+                            reinterpret_cast<const bool *>(masked.data()), b->list.size(), rows, b->image.data());
+    int64_t total = 0;
+    for (int f = 0; f < kBatchFamilies; ++f) total = std::max<int64_t>(total, b->layout.h[f].nunits);
+    if (total >= ((int64_t)1 << 31))
+        return fail(STRATA_EINVAL, "strata: a batch list of %lld work units exceeds the limit of 2^31",
+                    (long long)total);
+    const size_t bytes = b->layout.bytes;
     if (bytes > b->dev_bytes) {
         if (b->dev) HIP_TRY(hipFree(b->dev), "freeing a work list");
         b->dev = nullptr;
         HIP_TRY(hipMalloc((void **)&b->dev, bytes), "allocating a work list");
         b->dev_bytes = bytes;
     }
-    HIP_TRY(hipMemcpy(b->dev + oh, &b->hdr, sizeof b->hdr, hipMemcpyHostToDevice), "uploading a work list");
-    HIP_TRY(hipMemcpy(b->dev + oo, b->ops.data(), sizeof(BatchOp) * b->ops.size(), hipMemcpyHostToDevice),
-            "uploading a work list");
-    HIP_TRY(hipMemcpy(b->dev + ot, b->list.data(), sizeof(BatchTask) * b->list.size(), hipMemcpyHostToDevice),
-            "uploading a work list");
-    (void)oc;
+    HIP_TRY(hipMemcpy(b->dev, b->image.data(), bytes, hipMemcpyHostToDevice), "uploading a work list");
     return STRATA_OK;
 }
 
 hipError_t enqueue(strata_batch *b)
 {
-    const size_t oo = 256, ot = oo + ((sizeof(BatchOp) * b->ops.size() + 255) & ~(size_t)255),
-                 oc = ot + ((sizeof(BatchTask) * b->list.size() + 255) & ~(size_t)255);
-    return launch_batch(reinterpret_cast<const BatchHeader *>(b->dev), reinterpret_cast<const BatchTask *>(b->dev + ot),
-                        reinterpret_cast<const BatchOp *>(b->dev + oo), reinterpret_cast<unsigned *>(b->dev + oc),
-                        b->blocks, b->stream);
+    int max_blocks[kBatchFamilies];
+    for (int f = 0; f < kBatchFamilies; ++f)
+        max_blocks[f] = b->cus * (b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, batch_blocks_per_cu(f))
+                                                        : batch_blocks_per_cu(f));
+    const BatchLaunch L = launch_view(b->layout, b->dev, max_blocks);
+    return launch_batch(L, b->stream, b->side[0], b->side[1], b->ev);
 }
 
 int set_shape(strata_batch *b, int rows, int blocks_per_cu, bool graph)
 {
     if (rows < 4 || rows > 4096) return fail(STRATA_EINVAL, "strata: rows %d not in 4..4096", rows);
-    // Runs still in flight read the work list about to be rewritten.
+    // Runs still in flight read the work lists about to be rewritten.
     HIP_TRY(hipStreamSynchronize(b->stream), "waiting for a batch");
     int rc = build_list(b, rows);
     if (rc) return rc;
-    const int occ = batch_blocks_per_cu();
-    b->blocks_per_cu = blocks_per_cu > 0 ? std::min(blocks_per_cu, occ) : occ;
-    b->blocks = b->blocks_per_cu * b->cus;
+    b->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : 0;
+    b->blocks = b->cus * (blocks_per_cu > 0 ? std::min(blocks_per_cu, batch_blocks_per_cu(BF_HORN))
+                                            : batch_blocks_per_cu(BF_HORN));
     if (b->gx) HIP_TRY(hipGraphExecDestroy(b->gx), "destroying a graph");
     if (b->g) HIP_TRY(hipGraphDestroy(b->g), "destroying a graph");
     b->gx = nullptr;
@@ -295,6 +359,16 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
         }
         b->own_stream = true;
     }
+    for (hipStream_t &st : b->side)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            return fail(STRATA_EDEVICE, "strata: cannot create a stream");
+        }
+    for (hipEvent_t &e : b->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete b;
+            return fail(STRATA_EDEVICE, "strata: cannot create an event");
+        }
     const int rc = set_shape(b, opts && opts->rows ? opts->rows : 64, opts ? opts->blocks_per_cu : 0,
                              opts && opts->graph);
     if (rc) {
@@ -341,7 +415,7 @@ int strata_batch_tune(strata_batch *b, int reps, strata_batch_options *chosen)
     HIP_TRY(hipEventCreate(&e1), "creating an event");
     float best = 0.0f;
     int best_rows = b->rows, best_bpc = b->blocks_per_cu;
-    const int occ = batch_blocks_per_cu();
+    const int occ = batch_blocks_per_cu(BF_HORN);
     const int rows_c[] = {16, 32, 64, 128};
     std::vector<int> bpc_c = {occ};
     if (occ > 1) bpc_c.push_back(std::max(1, occ / 2));
@@ -393,8 +467,8 @@ int strata_batch_info_get(const strata_batch *b, strata_batch_info *out)
     memset(out, 0, sizeof *out);
     out->tasks = (int64_t)b->tasks.size();
     out->cells = b->cells;
-    out->data_units = b->hdr.ndata;
-    out->validity_units = b->hdr.nval;
+    out->data_units = b->layout.h[BF_HORN].nunits + b->layout.h[BF_FOCAL].nunits;
+    out->validity_units = b->layout.h[BF_VALIDITY].nunits;
     out->rows = b->rows;
     out->blocks = b->blocks;
     out->graph = b->graph ? 1 : 0;

@@ -32,10 +32,28 @@ namespace strata {
 // Shared helpers
 // ---------------------------------------------------------------------------
 
+// Rasters, masks and work lists live in global memory. Pointers that reach
+// a kernel inside a structure (a work list) are generic to the compiler,
+// which would then use flat loads and stores (waiting on both the vector
+// and the scalar memory counters); accessing through these global-typed
+// pointers gives global instructions, as for pointers in kernel arguments.
+#define STRATA_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ STRATA_GLOBAL T *gp(T *p)
+{
+    return (STRATA_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ const STRATA_GLOBAL T *gp(const T *p)
+{
+    return (const STRATA_GLOBAL T *)p;
+}
+typedef float pair_t __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float load_cell(const SrcView &s, int64_t x, int64_t y)
 {
     bool in = x >= s.lx0 && x < s.lx1 && y >= s.ly0 && y < s.ly1;
-    return in ? s.p[(y - s.sy) * s.stride + (x - s.sx)] : 0.0f;
+    return in ? gp(s.p)[(y - s.sy) * s.stride + (x - s.sx)] : 0.0f;
 }
 
 // v from lane-1; lane 0 gets `edge` (the wave's left halo column).
@@ -255,14 +273,16 @@ struct NoPost {
 // PAIRS: every pair of the block lies inside the buffer and the region, at
 // 8-byte-aligned addresses, so rows move as dwordx2 loads and stores. `post`
 // maps every stored value (edge NaNs included) of the first output.
+// tid: the lane's index among the lanes sharing columns from xb (a block's
+// 256 or a wave's 64); each lane takes columns xb + 2*tid and xb + 2*tid + 1.
 template <int OP, bool PAIRS, class Post = NoPost>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
-                                      const DstView &d1, int64_t xb, int64_t y0, int64_t y1,
+                                      const DstView &d1, int64_t xb, int64_t y0, int64_t y1, uint32_t tid,
                                       const Post &post = Post())
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
-    const int lane = threadIdx.x & 63;
-    const uint32_t off = 2u * threadIdx.x;  // the pair's first column, relative to xb
+    const int lane = (int)(tid & 63);
+    const uint32_t off = 2u * tid;  // the pair's first column, relative to xb
     const int64_t x = xb + off;
     const bool halo_lane = lane == 0 || lane == 63;
     // Offsets from a row's start of the cells this lane loads, clamped into
@@ -277,10 +297,10 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     const bool in1 = x + 1 >= rg.ox && x + 1 < rg.ox + rg.w;
 
     auto load = [&](Row &row, int64_t y) {
-        const float *rp = s.p + (min(max(y, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
+        const STRATA_GLOBAL float *rp = gp(s.p) + (min(max(y, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
         float c0, c1, h = 0.0f;
         if constexpr (PAIRS) {
-            const float2 v = *reinterpret_cast<const float2 *>(rp + o0);
+            const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0);
             c0 = v.x;
             c1 = v.y;
         } else {
@@ -300,9 +320,9 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
         row.hd1 = row.r - row.c0;
     };
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
-        float *rp = d.p + (y - rg.oy) * d.stride;
+        STRATA_GLOBAL float *rp = gp(d.p) + (y - rg.oy) * d.stride;
         if constexpr (PAIRS) {
-            *reinterpret_cast<float2 *>(rp + so) = make_float2(v0, v1);
+            *reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so) = pair_t{v0, v1};
         } else {
             if (in0) rp[so] = v0;
             if (in1) rp[so + 1] = v1;
@@ -356,9 +376,9 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
                        xb + kSweepCols <= rg.ox + rg.w;
     if (pairs)
-        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1);
+        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
     else
-        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1);
+        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -577,10 +597,10 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         const int64_t y = ty0 + rbase + o;
         if (!col_in || y >= rg.oy + rg.h) continue;
         const bool interior = col_interior && y >= R && y < rg.H - R;
-        d0.p[(y - rg.oy) * d0.stride + (x - rg.ox)] = interior ? out0[o] : edge_nan();
+        gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = interior ? out0[o] : edge_nan();
         if constexpr (KIND == FK_FIT) {
             if (P.op == STRATA_OP_GRADIENT)
-                d1.p[(y - rg.oy) * d1.stride + (x - rg.ox)] = interior ? out1[o] : edge_nan();
+                gp(d1.p)[(y - rg.oy) * d1.stride + (x - rg.ox)] = interior ? out1[o] : edge_nan();
         }
     }
 }
@@ -614,7 +634,7 @@ __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, Src
     const int64_t y0 = rg.oy + (int64_t)blockIdx.y * kPointRows;
     const int64_t y1 = min(y0 + (int64_t)kPointRows, rg.oy + rg.h);
     for (int64_t y = y0; y < y1; ++y) {
-        const float v = s.p[(y - s.sy) * s.stride + (x - s.sx)];
+        const float v = gp(s.p)[(y - s.sy) * s.stride + (x - s.sx)];
         float o;
         if constexpr (OP == STRATA_OP_LOOKUP) {
             const int n = P.ntaps / 2;
@@ -623,7 +643,7 @@ __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, Src
             const int nb = (P.ntaps - 1) / 2;
             o = reclass_of(v, P.taps, nb, P.taps + nb);
         }
-        d0.p[(y - rg.oy) * d0.stride + (x - rg.ox)] = o;
+        gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = o;
     }
 }
 
@@ -707,7 +727,7 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
         if (xc >= src.lx0 && xc + 64 * 32 <= src.lx1 && y0 - R >= src.ly0 && y0 + kVsRows + R <= src.ly1) {
 #pragma unroll
             for (int j = 0; j < NL; ++j) {
-                const uint32_t *rowp = sbits + (p >> 5);
+                const STRATA_GLOBAL uint32_t *rowp = gp(sbits) + (p >> 5);
                 Wd[j] = rowp[lane];
                 sh[j] = (uint32_t)(p & 31);
                 p += src.stride;
@@ -725,7 +745,7 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
                 const int64_t base = p >> 5;  // word of lane 0
                 const int64_t lo = ((p + dlo) >> 5) - base, hi = ((p + dhi) >> 5) - base;
                 const bool ok = yy >= src.ly0 && yy < src.ly1 && lo <= 63 && hi >= 0;
-                const uint32_t *rowp = sbits + (ok ? base : safe);
+                const STRATA_GLOBAL uint32_t *rowp = gp(sbits) + (ok ? base : safe);
                 const int32_t lo32 = ok ? clamp32(lo, 0, 63) : 0, hi32 = ok ? clamp32(hi, 0, 63) : 0;
                 const int32_t lc = min(max(lane, lo32), hi32);
                 const uint32_t w = rowp[lc];
@@ -769,7 +789,8 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
         v &= imask;
         const uint32_t s = (uint32_t)(ob & 31);
         const int64_t wb = (ob >> 5) + wx * kVsOut - 2;
-        uint32_t *rowd = dbits + wb;
+        STRATA_GLOBAL uint32_t *rowd = gp(dbits) + wb;
+        uint32_t *rowd_flat = dbits + wb;  // for the atomics
         const uint32_t prev = lane_down(v);
         const uint32_t out = s == 0 ? v : __builtin_amdgcn_alignbit(v, prev, 32u - s);
         if (whole) {
@@ -785,8 +806,8 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
                 if (own == ~0u) {
                     rowd[lane] = out;
                 } else {
-                    atomicAnd(rowd + lane, ~own | (out & own));
-                    atomicOr(rowd + lane, out & own);
+                    atomicAnd(rowd_flat + lane, ~own | (out & own));
+                    atomicOr(rowd_flat + lane, out & own);
                 }
             }
         }
@@ -809,22 +830,30 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
 }
 
 // ---------------------------------------------------------------------------
-// batch_kernel: many tiles and mixed ops in one persistent launch
+// batch_kernel: many tiles and mixed ops, persistent launches over work lists
 // ---------------------------------------------------------------------------
 //
-// Workgroups claim units from a counter in order (data units of every task,
-// then validity units) until the list is empty, so uneven tiles balance
-// across the chip and one launch (capturable in a graph) runs the whole
-// batch. The task of a unit is found by binary search over the tasks' first
-// units; the unit index is read back through readfirstlane so the search and
-// the task's fields are scalar.
+// A batch's work is split into three lists: radius-1 sweep units, focal
+// tile units and validity units. Each list is run by its own persistent
+// kernel (batch_kernel<FAMILY>), so each compiles to the registers its own
+// work needs and keeps its occupancy; the three run side by side. Within a
+// list, workgroups claim units from a counter in order until the list is
+// empty, so uneven tiles balance across the chip. The next claim is issued
+// while the current unit runs, and the unit's task is found by binary
+// search over the tasks' first units unless it is the previous unit's task.
+// Unit indices are read back through readfirstlane, so the search and the
+// task's fields are scalar.
 
-// The point ops of a batch op, applied to a stencil op's output values.
+// The point ops of a batch op, applied to a stencil op's output values. The
+// count is read once: the sweep's stores could alias the op table as far as
+// the compiler knows, which would reload it after every store.
 struct ChainPost {
     const BatchOp *op;
+    int npost;
+    __device__ explicit ChainPost(const BatchOp *o) : op(o), npost(o->npost) {}
     __device__ float operator()(float v) const
     {
-        for (int i = 0; i < op->npost; ++i) {
+        for (int i = 0; i < npost; ++i) {
             const float *t = op->post_taps[i];
             if (op->post_op[i] == STRATA_OP_LOOKUP) {
                 const int n = op->post_n[i] / 2;
@@ -840,6 +869,10 @@ struct ChainPost {
 
 constexpr int kBatchMaxFocalR = 3;
 constexpr int kBatchLdsFloats = focal_lds_floats<FK_MEAN, kBatchMaxFocalR>();
+constexpr int kFocalTilesPerUnit = 4;  // focal tiles stacked in one unit
+
+// One wave's unit: 128 columns (two per lane) of `rows` rows.
+constexpr int kWaveCols = 128;
 
 template <int OP>
 __device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t, int64_t lu)
@@ -851,16 +884,30 @@ __device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t
     a.aligned = t.aligned;
     a.rows = (int32_t)t.rows;
     a.blocks_x = t.blocks_x;
-    const int64_t xb = t.rg.ox + (lu % t.blocks_x) * kSweepCols;
+    const int64_t xb = t.rg.ox + (lu % t.blocks_x) * kWaveCols;
     const int64_t y0 = t.rg.oy + (lu / t.blocks_x) * t.rows;
     const int64_t y1 = min(y0 + (int64_t)t.rows, t.rg.oy + t.rg.h);
-    const bool pairs = t.aligned && xb >= t.s.lx0 && xb + kSweepCols <= t.s.lx1 && xb >= t.rg.ox &&
-                       xb + kSweepCols <= t.rg.ox + t.rg.w;
-    const ChainPost post{&op};
+    const bool pairs = t.aligned && xb >= t.s.lx0 && xb + kWaveCols <= t.s.lx1 && xb >= t.rg.ox &&
+                       xb + kWaveCols <= t.rg.ox + t.rg.w;
+    const ChainPost post(&op);
+    const uint32_t lane = threadIdx.x & 63;
     if (pairs)
-        sweep<OP, true>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, post);
+        sweep<OP, true>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
     else
-        sweep<OP, false>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, post);
+        sweep<OP, false>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
+}
+
+template <int KIND, int R>
+__device__ __forceinline__ void batch_focal_tiles(const FocalArgs &a, const BatchTask &t, int64_t lu, float *lds)
+{
+    const PtrTaps taps{nullptr};
+    const int64_t tx0 = t.rg.ox + (lu % t.blocks_x) * kTileX;
+    const int64_t ty = t.rg.oy + (lu / t.blocks_x) * t.rows;
+    for (int k = 0; k < kFocalTilesPerUnit; ++k) {
+        const int64_t ty0 = ty + k * kTileY;
+        if (ty0 >= t.rg.oy + t.rg.h) break;
+        focal_block<KIND, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds);
+    }
 }
 
 template <int R>
@@ -871,25 +918,33 @@ __device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &
     a.flags = op.flags;
 #pragma unroll
     for (int i = 0; i < 8; ++i) a.k[i] = op.k[i];
-    const PtrTaps taps{nullptr};
-    const int64_t tx0 = t.rg.ox + (lu % t.blocks_x) * kTileX;
-    const int64_t ty0 = t.rg.oy + (lu / t.blocks_x) * kTileY;
     switch (op.kind) {
-    case BK_MEAN: focal_block<FK_MEAN, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
-    case BK_MIN: focal_block<FK_MIN, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
-    default: focal_block<FK_MAX, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds); break;
+    case BK_MEAN: batch_focal_tiles<FK_MEAN, R>(a, t, lu, lds); break;
+    case BK_MIN: batch_focal_tiles<FK_MIN, R>(a, t, lu, lds); break;
+    default: batch_focal_tiles<FK_MAX, R>(a, t, lu, lds); break;
     }
 }
 
-// The last task whose first unit (u0 or v0) is <= u.
-template <bool VALIDITY>
+// One validity wave of a task per unit.
+__device__ __forceinline__ void batch_validity(const BatchOp &op, const BatchTask &t, int64_t w)
+{
+    const int64_t wx = w % t.vwaves_x, strip = w / t.vwaves_x;
+    const int64_t y0 = t.rg.oy + strip * kVsRows;
+    const int64_t y1 = min(y0 + (int64_t)kVsRows, t.rg.oy + t.rg.h);
+    switch (op.radius) {
+    case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+    case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+    default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+    }
+}
+
+// The last task whose first unit is <= u.
 __device__ __forceinline__ int64_t batch_find(const BatchTask *tasks, int64_t n, int64_t u)
 {
     int64_t lo = 0, hi = n - 1;
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) >> 1;
-        const int64_t first = VALIDITY ? tasks[mid].v0 : tasks[mid].u0;
-        if (first <= u)
+        if (tasks[mid].u0 <= u)
             lo = mid;
         else
             hi = mid - 1;
@@ -897,25 +952,71 @@ __device__ __forceinline__ int64_t batch_find(const BatchTask *tasks, int64_t n,
     return lo;
 }
 
-__global__ __launch_bounds__(256) void batch_clear(unsigned *counter) { *counter = 0u; }
+// Each list's counter sits on its own 256-byte line.
+constexpr int kCounterStride = 64;  // unsigned words
 
-__global__ __launch_bounds__(256) void batch_kernel(const BatchHeader *hdr, const BatchTask *tasks,
-                                                    const BatchOp *ops, unsigned *counter)
+__global__ __launch_bounds__(64) void batch_clear(unsigned *counters)
 {
-    __shared__ float lds[kBatchLdsFloats];
-    __shared__ unsigned claimed;
-    const int64_t ntasks = hdr->ntasks, ndata = hdr->ndata, nval = hdr->nval;
-    for (;;) {
-        __syncthreads();  // every thread has read the previous claim
+    if (threadIdx.x < kBatchFamilies) counters[threadIdx.x * kCounterStride] = 0u;
+}
+
+// Sweep and validity units belong to one wave: each wave claims its own
+// (lane 0's atomic, broadcast with readfirstlane) and never waits for the
+// rest of its workgroup. Focal units use the workgroup's LDS tile: the
+// workgroup claims them together.
+// The lists and ops are read-only for the whole launch and never alias the
+// rasters the units write (__restrict__), so their fields stay in registers
+// across the units' stores instead of being reloaded after each one.
+template <int FAM>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchHeader *__restrict__ hdr,
+                                                    const BatchTask *__restrict__ tasks,
+                                                    const BatchOp *__restrict__ ops, unsigned *counter)
+{
+    const int64_t ntasks = hdr->ntasks, nunits = hdr->nunits;
+    int64_t ti = 0, tu0 = 0, tu1 = 0;  // the current task and its units [tu0, tu1)
+    if constexpr (FAM == BF_FOCAL) {
+        __shared__ float lds[kBatchLdsFloats];
+        __shared__ unsigned claimed;
         if (threadIdx.x == 0) claimed = atomicAdd(counter, 1u);
         __syncthreads();
-        const int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
-        if (u >= ndata + nval) return;
-        if (u < ndata) {
-            const BatchTask &t = tasks[batch_find<false>(tasks, ntasks, u)];
+        int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+        while (u < nunits) {
+            unsigned next = 0u;
+            if (threadIdx.x == 0) next = atomicAdd(counter, 1u);  // consumed after this unit
+            if (u < tu0 || u >= tu1) {
+                ti = batch_find(tasks, ntasks, u);
+                tu0 = tasks[ti].u0;
+                tu1 = tasks[ti].u1;
+            }
+            const BatchTask &t = tasks[ti];
             const BatchOp &op = ops[t.op];
-            const int64_t lu = u - t.u0;
-            if (op.kind == BK_HORN) {
+            switch (op.radius) {
+            case 1: batch_focal<1>(op, t, u - tu0, lds); break;
+            case 2: batch_focal<2>(op, t, u - tu0, lds); break;
+            default: batch_focal<3>(op, t, u - tu0, lds); break;
+            }
+            __syncthreads();  // the unit is done (and the LDS free) on every wave
+            if (threadIdx.x == 0) claimed = next;
+            __syncthreads();
+            u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+        }
+    } else {
+        const bool lead = (threadIdx.x & 63) == 0;
+        unsigned c = 0u;
+        if (lead) c = atomicAdd(counter, 1u);
+        int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(c);
+        while (u < nunits) {
+            unsigned next = 0u;
+            if (lead) next = atomicAdd(counter, 1u);  // consumed after this unit
+            if (u < tu0 || u >= tu1) {
+                ti = batch_find(tasks, ntasks, u);
+                tu0 = tasks[ti].u0;
+                tu1 = tasks[ti].u1;
+            }
+            const BatchTask &t = tasks[ti];
+            const BatchOp &op = ops[t.op];
+            const int64_t lu = u - tu0;
+            if constexpr (FAM == BF_HORN) {
                 switch (op.op) {
                 case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu); break;
                 case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu); break;
@@ -923,26 +1024,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchHeader *hdr, cons
                 default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu); break;
                 }
             } else {
-                switch (op.radius) {
-                case 1: batch_focal<1>(op, t, lu, lds); break;
-                case 2: batch_focal<2>(op, t, lu, lds); break;
-                default: batch_focal<3>(op, t, lu, lds); break;
-                }
+                batch_validity(op, t, lu);
             }
-        } else {
-            const int64_t vu = u - ndata;
-            const BatchTask &t = tasks[batch_find<true>(tasks, ntasks, vu)];
-            const int64_t w = (vu - t.v0) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            if (w < t.vunits) {
-                const int64_t wx = w % t.vwaves_x, strip = w / t.vwaves_x;
-                const int64_t y0 = t.rg.oy + strip * kVsRows;
-                const int64_t y1 = min(y0 + (int64_t)kVsRows, t.rg.oy + t.rg.h);
-                switch (ops[t.op].radius) {
-                case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-                case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-                default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-                }
-            }
+            u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(next);
         }
     }
 }
@@ -1115,27 +1199,59 @@ void batch_units(const BatchOp &op, BatchTask &t, int rows)
 {
     if (op.kind == BK_HORN) {
         t.rows = (uint32_t)rows;
-        t.blocks_x = (uint32_t)cdiv(t.rg.w, kSweepCols);
+        t.blocks_x = (uint32_t)cdiv(t.rg.w, kWaveCols);
     } else {
-        t.rows = kTileY;
+        t.rows = kTileY * kFocalTilesPerUnit;
         t.blocks_x = (uint32_t)cdiv(t.rg.w, kTileX);
     }
     t.vwaves_x = (uint32_t)cdiv(t.rg.w / 32 + 2, kVsOut);
 }
 
-hipError_t launch_batch(const BatchHeader *hdr, const BatchTask *tasks, const BatchOp *ops, unsigned *counter,
-                        int blocks, hipStream_t st)
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side0, hipStream_t side1,
+                        hipEvent_t *ev)
 {
-    hipLaunchKernelGGL(batch_clear, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL(batch_kernel, dim3((unsigned)blocks), dim3(256), 0, st, hdr, tasks, ops, counter);
-    return hipGetLastError();
+    hipLaunchKernelGGL(batch_clear, dim3(1), dim3(64), 0, st, L.counters);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const bool fork = side0 && side1 && ev;
+    if (fork) {
+        // The focal and validity lists run on two side streams, joined back
+        // before the batch counts as done (also inside a graph capture).
+        if ((e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side0, ev[0], 0)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side1, ev[0], 0)) != hipSuccess) return e;
+    }
+    hipStream_t on[kBatchFamilies] = {st, fork ? side0 : st, fork ? side1 : st};
+    if (L.nunits[BF_HORN] > 0)
+        hipLaunchKernelGGL(batch_kernel<BF_HORN>, dim3((unsigned)L.blocks[BF_HORN]), dim3(256), 0, on[BF_HORN],
+                           L.hdr[BF_HORN], L.tasks[BF_HORN], L.ops, L.counters + BF_HORN * kCounterStride);
+    if (L.nunits[BF_FOCAL] > 0)
+        hipLaunchKernelGGL(batch_kernel<BF_FOCAL>, dim3((unsigned)L.blocks[BF_FOCAL]), dim3(256), 0, on[BF_FOCAL],
+                           L.hdr[BF_FOCAL], L.tasks[BF_FOCAL], L.ops, L.counters + BF_FOCAL * kCounterStride);
+    if (L.nunits[BF_VALIDITY] > 0)
+        hipLaunchKernelGGL(batch_kernel<BF_VALIDITY>, dim3((unsigned)L.blocks[BF_VALIDITY]), dim3(256), 0,
+                           on[BF_VALIDITY], L.hdr[BF_VALIDITY], L.tasks[BF_VALIDITY], L.ops,
+                           L.counters + BF_VALIDITY * kCounterStride);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (fork) {
+        if ((e = hipEventRecord(ev[1], side0)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev[2], side1)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(st, ev[1], 0)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(st, ev[2], 0)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
-int batch_blocks_per_cu()
+int batch_blocks_per_cu(int family)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 256, 0) != hipSuccess || n < 1) n = 1;
-    return n;
+    hipError_t e = hipErrorInvalidValue;
+    switch (family) {
+    case BF_HORN: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_HORN>, 256, 0); break;
+    case BF_FOCAL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_FOCAL>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_VALIDITY>, 256, 0); break;
+    }
+    return (e != hipSuccess || n < 1) ? 1 : n;
 }
 
 } // namespace strata

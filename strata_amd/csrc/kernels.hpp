@@ -104,9 +104,11 @@ struct BatchOp {
 };
 
 // One tile: an op over its own raster (its edge ring is its own), written to
-// its own output. Data work is split into units of one 512-column strip of
-// `rows` rows (radius-1 ops) or one focal tile; validity work into units of
-// four validity waves.
+// its own output. A task sits in two work lists: its data list (units of one
+// wave's 128 columns of `rows` rows for radius-1 ops, or a workgroup's column
+// of focal tiles) and, when its output has a mask, the validity list (units
+// of one validity wave). [u0, u1) are its units in the list the copy belongs
+// to.
 struct BatchTask {
     Region rg;
     SrcView s;
@@ -117,27 +119,47 @@ struct BatchTask {
     int32_t vunits;   // validity waves (0 when the output has no mask)
     int32_t aligned;  // pair (8-byte) accesses allowed
     uint32_t blocks_x;  // data units across
-    uint32_t rows;      // rows per data unit (radius-1 ops)
+    uint32_t rows;      // rows per data unit
     uint32_t vwaves_x;  // validity waves across
     uint32_t pad;
-    int64_t u0;  // first data unit
-    int64_t v0;  // first validity unit (four waves each)
+    int64_t u0, u1;
 };
+
+// Work lists of a batch, one persistent kernel each.
+enum BatchFamily : int {
+    BF_HORN = 0,      // radius-1 terrain sweeps
+    BF_FOCAL = 1,     // focal tiles
+    BF_VALIDITY = 2,  // validity of every masked output
+};
+constexpr int kBatchFamilies = 3;
 
 struct BatchHeader {
-    int64_t ntasks, ndata, nval;
+    int64_t ntasks, nunits;
 };
 
-// Units of each kind a task needs (data units and validity units of four
-// waves); rows is the strip height of radius-1 ops.
+// Device pointers of one batch's lists (headers, tasks sorted by u0, one
+// claim counter each) and the launch shape.
+struct BatchLaunch {
+    const BatchOp *ops;
+    const BatchHeader *hdr[kBatchFamilies];
+    const BatchTask *tasks[kBatchFamilies];
+    unsigned *counters;  // kBatchFamilies counters
+    int64_t nunits[kBatchFamilies];
+    int blocks[kBatchFamilies];
+};
+
+// Fills a task's unit shape for `rows` rows per radius-1 unit: rows,
+// blocks_x (data units across) and vwaves_x.
 void batch_units(const BatchOp &op, BatchTask &t, int rows);
 
-// Clears the claim counter (a kernel, so it can live in a captured graph)
-// and runs the persistent batch kernel with `blocks` workgroups.
-hipError_t launch_batch(const BatchHeader *hdr, const BatchTask *tasks, const BatchOp *ops, unsigned *counter,
-                        int blocks, hipStream_t st);
+// Clears the claim counters (a kernel, so it can live in a captured graph)
+// and runs the lists' persistent kernels: on st alone, or, given two side
+// streams and three events, the focal and validity lists on the side
+// streams joined back into st.
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side0, hipStream_t side1,
+                        hipEvent_t *ev);
 
-// Workgroups of the batch kernel resident per CU (occupancy), for sizing.
-int batch_blocks_per_cu();
+// Workgroups of a list's kernel resident per CU (occupancy), for sizing.
+int batch_blocks_per_cu(int family);
 
 } // namespace strata
