@@ -13,7 +13,9 @@ kernel) from HIP events the library records around each of its launches
 on the launch stream, the rest of the terrain suite (Aspect, Hillshade,
 profile/plan/mean Curvature, Gradient) on the same DEM, a batch of 4096
 uneven mixed-op tiles run as one persistent graph-captured launch against
-one launch set per tile, and the CPU baseline: the oracle (a scalar C restatement of the reference's kernels)
+one launch set per tile, a serving sweep (Poisson arrivals at fixed QPS,
+p50/p99 latency of the chain Slope -> Lookup -> Reclass on window
+requests), and the CPU baseline: the oracle (a scalar C restatement of the reference's kernels)
 timed on the host cores over a bounded sample of the same rows, whose
 output is also compared bit for bit with the GPU's.
 
@@ -55,6 +57,13 @@ def parse():
     ap.add_argument("--batch-tiles", type=int, default=4096,
                     help="tiles in the batched-launch comparison (0: skip it)")
     ap.add_argument("--batch-reps", type=int, default=10, help="timed runs of the batch")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrival time per QPS level of the serving sweep (0: skip it)")
+    ap.add_argument("--serve-qps", type=str, default="100,200,500,1000,2000,5000",
+                    help="QPS levels of the serving sweep")
+    ap.add_argument("--serve-rows", type=int, default=0,
+                    help="rows per sweep unit when serving (0: chosen per request; short units run a "
+                         "small request's rows in parallel)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     return ap.parse_args()
@@ -175,6 +184,10 @@ def main() -> int:
         torch.cuda.empty_cache()
         batch = batch_bench(args, dem, N, dev, log)
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_bench(args, dem, N, log)
+
     if args.calibrate_copy:
         tmp = torch.empty_like(dem_data)
         tmp.copy_(dem_data)
@@ -210,7 +223,13 @@ def main() -> int:
         "parity_sample": parity,
         "suite": suite,
         "batch": batch,
+        "serving": serving,
     }
+    if serving and serving["levels"]:
+        at = min(serving["levels"], key=lambda lv: abs(lv["qps"] - 1000.0))
+        line["latency"] = {"qps": at["qps"], "p50_ms": at["p50_ms"], "p99_ms": at["p99_ms"],
+                           "what": "serving chain Slope -> Lookup -> Reclass per window request, "
+                                   "arrival to completion"}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -303,6 +322,54 @@ def batch_bench(args, dem, N, dev, log):
     del out, mask
     torch.cuda.empty_cache()
     return res
+
+
+def serving_bench(args, dem, N, log):
+    """Open-loop Poisson arrivals at each QPS level against the resident
+    DEM; each request is the chain Slope -> Lookup -> Reclass on a window
+    (80% 512², 15% 2048², 5% 4096²). Latency runs from each request's
+    scheduled arrival to its completion."""
+    import numpy as np
+
+    from strata_amd import terrain, transfer
+    from strata_amd.serve import Server, poisson_schedule
+
+    slope = terrain.slopeOp(terrain.SlopeOptions(CellSize=30))
+    lookup = transfer.newLookupKernel("risk", [0, 5, 10, 20, 30, 40], [1, 1.1, 1.3, 1.9, 3, 4.5])
+    reclass = transfer.newReclassKernel("class", [1.2, 1.6, 2.2, 3], [1, 2, 3, 4, 5])
+    budget = 8 << 30
+    levels = []
+    with Server(dem, slope, (lookup, reclass), slots=4, hbm_budget=budget, max_batch=256,
+                rows=args.serve_rows, auto_release=True) as srv:
+        w, a = poisson_schedule(500.0, 100, N, N, seed=99)
+        srv.replay(w, a)  # warm-up
+        for i, qps in enumerate(float(q) for q in args.serve_qps.split(",")):
+            n = int(max(150, qps * args.serve_seconds))
+            windows, arrivals = poisson_schedule(qps, n, N, N, seed=100 + i)
+            before = srv.stats()
+            lat = srv.replay(windows, arrivals)
+            after = srv.stats()
+            cells = int((windows[:, 2] * windows[:, 3]).sum())
+            span = float(arrivals[-1] + lat[-1] / 1e3) if n else 0.0
+            batches = after["batches"] - before["batches"]
+            lv = {"qps": qps, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 4), "mean_ms": round(float(lat.mean()), 4),
+                  "max_ms": round(float(lat.max()), 4), "achieved_rps": round(n / span, 1),
+                  "gcells_s": round(cells / span / 1e9, 3), "batches": batches,
+                  "mean_batch": round(n / max(1, batches), 2),
+                  "admission_waits": after["admission_waits"] - before["admission_waits"],
+                  "mean_queue_ms": round((after["queue_ms"] - before["queue_ms"]) / n, 4),
+                  "mean_service_ms": round((after["service_ms"] - before["service_ms"]) / n, 4)}
+            levels.append(lv)
+            log(f"bench: serve {qps:.0f} QPS: p50 {lv['p50_ms']} ms, p99 {lv['p99_ms']} ms, "
+                f"{lv['achieved_rps']} req/s, mean batch {lv['mean_batch']}, queue {lv['mean_queue_ms']} ms, "
+                f"service {lv['mean_service_ms']} ms")
+        st = srv.stats()
+    return {"workload": f"open-loop Poisson arrivals; windows 80% 512², 15% 2048², 5% 4096² of the resident "
+                        f"{N}x{N} DEM; chain Slope (CellSize 30, degrees) -> Lookup (0,5,10,20,30,40 -> "
+                        f"1,1.1,1.3,1.9,3,4.5) -> Reclass (1.2,1.6,2.2,3 -> 1..5) fused in one pass",
+            "slots": 4, "hbm_budget_bytes": budget, "max_batch": 256, "rows": args.serve_rows, "levels": levels,
+            "peak_result_bytes": st["peak_bytes"], "max_in_flight": st["max_in_flight"]}
 
 
 def cpu_baseline(args, dem, out, slope, N, log):

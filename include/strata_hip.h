@@ -357,7 +357,8 @@ typedef struct strata_serve_config {
     int32_t slots;         /* batches in flight at once; 0: 4             */
     int64_t hbm_budget;    /* bytes of result memory; 0: 4 GiB            */
     int32_t max_batch;     /* requests per batch; 0: 256                  */
-    int32_t rows;          /* rows per work unit of radius-1 ops; 0: 64   */
+    int32_t rows;          /* rows per work unit of radius-1 ops; 0: per
+                              request, short units for small windows      */
     int32_t auto_release;  /* 1: drop each result when it completes       */
     int32_t reserved;
 } strata_serve_config;
@@ -374,7 +375,8 @@ typedef struct strata_serve_stats {
     int64_t admission_waits; /* times the queue head waited for memory    */
     int64_t peak_bytes;    /* peak result memory in use                   */
     double busy_ms;        /* wall time with at least one batch in flight */
-    double reserved;
+    double queue_ms;       /* sum over requests of arrival -> launch      */
+    double service_ms;     /* sum over requests of launch -> completion   */
 } strata_serve_stats;
 
 typedef struct strata_server strata_server;

@@ -192,7 +192,8 @@ class ServeStats(ctypes.Structure):
         ("admission_waits", c_int64),
         ("peak_bytes", c_int64),
         ("busy_ms", ctypes.c_double),
-        ("reserved", ctypes.c_double),
+        ("queue_ms", ctypes.c_double),
+        ("service_ms", ctypes.c_double),
     ]
 
 

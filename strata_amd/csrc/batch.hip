@@ -32,8 +32,8 @@ struct strata_batch {
     size_t dev_bytes = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipStream_t side[2] = {nullptr, nullptr};  // the focal and validity lists' streams
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipStream_t side = nullptr;  // the focal list's stream
+    hipEvent_t ev[2] = {nullptr, nullptr};
     int rows = 64, blocks = 0, blocks_per_cu = 0, cus = 1;
     bool graph = false;
     hipGraph_t g = nullptr;
@@ -47,8 +47,7 @@ struct strata_batch {
         if (dev) (void)hipFree(dev);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t st : side)
-            if (st) (void)hipStreamDestroy(st);
+        if (side) (void)hipStreamDestroy(side);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -164,6 +163,18 @@ size_t list_bytes(int nops, size_t ntasks)
     return 256 + align256(sizeof(BatchOp) * (size_t)nops) + 3 * align256(sizeof(BatchTask) * ntasks) + 256 * kBatchFamilies;
 }
 
+// Rows per radius-1 unit that split one task into about kLatencyUnits wave
+// units (a power of two in [4, 64]): a small window then runs as many short
+// units at once instead of a few long row chains.
+constexpr int64_t kLatencyUnits = 8192;
+int adaptive_rows(const BatchTask &t)
+{
+    const int64_t across = cdiv64(t.rg.w, 128);
+    int rows = 4;
+    while (rows < 64 && across * cdiv64(t.rg.h, rows) > kLatencyUnits) rows *= 2;
+    return rows;
+}
+
 ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, const bool *masked, size_t n,
                        int rows, char *out)
 {
@@ -188,7 +199,7 @@ ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, con
     for (int f = 0; f < kBatchFamilies; ++f) lt[f] = reinterpret_cast<BatchTask *>(out + L.tasks[f]);
     for (size_t i = 0; i < n; ++i) {
         BatchTask t = tasks[i];
-        batch_units(ops[t.op], t, rows);
+        batch_units(ops[t.op], t, rows > 0 ? rows : adaptive_rows(t));
         const int64_t data = (int64_t)t.blocks_x * cdiv64(t.rg.h, t.rows);
         t.vunits = masked[i] ? (int32_t)((int64_t)t.vwaves_x * cdiv64(t.rg.h, 16)) : 0;
         const int f = ops[t.op].kind == BK_HORN ? BF_HORN : BF_FOCAL;
@@ -210,17 +221,20 @@ ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, con
     return L;
 }
 
-BatchLaunch launch_view(const ListLayout &L, char *dev, const int *max_blocks)
+BatchLaunch launch_view(const ListLayout &L, char *dev, int max_wave_blocks, int max_focal_blocks, bool clear)
 {
     BatchLaunch b;
     b.ops = reinterpret_cast<const BatchOp *>(dev + L.ops);
+    b.hdr = reinterpret_cast<const BatchHeader *>(dev);
     for (int f = 0; f < kBatchFamilies; ++f) {
-        b.hdr[f] = reinterpret_cast<const BatchHeader *>(dev) + f;
         b.tasks[f] = reinterpret_cast<const BatchTask *>(dev + L.tasks[f]);
         b.nunits[f] = L.h[f].nunits;
-        b.blocks[f] = (int)std::max<int64_t>(1, std::min<int64_t>(max_blocks[f], L.h[f].nunits));
     }
+    const int64_t waves = L.h[BF_HORN].nunits + L.h[BF_VALIDITY].nunits;
+    b.blocks_wave = (int)std::max<int64_t>(1, std::min<int64_t>(max_wave_blocks, cdiv64(waves, 4)));
+    b.blocks_focal = (int)std::max<int64_t>(1, std::min<int64_t>(max_focal_blocks, L.h[BF_FOCAL].nunits));
     b.counters = reinterpret_cast<unsigned *>(dev + L.counters);
+    b.clear = clear;
     return b;
 }
 
@@ -255,12 +269,13 @@ int build_list(strata_batch *b, int rows)
 
 hipError_t enqueue(strata_batch *b)
 {
-    int max_blocks[kBatchFamilies];
-    for (int f = 0; f < kBatchFamilies; ++f)
-        max_blocks[f] = b->cus * (b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, batch_blocks_per_cu(f))
-                                                        : batch_blocks_per_cu(f));
-    const BatchLaunch L = launch_view(b->layout, b->dev, max_blocks);
-    return launch_batch(L, b->stream, b->side[0], b->side[1], b->ev);
+    auto cap = [&](int family) {
+        const int occ = batch_blocks_per_cu(family);
+        return b->cus * (b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, occ) : occ);
+    };
+    // Replays reuse the uploaded lists: the counters are cleared by a kernel.
+    const BatchLaunch L = launch_view(b->layout, b->dev, cap(BF_HORN), cap(BF_FOCAL), true);
+    return launch_batch(L, b->stream, b->side, b->ev);
 }
 
 int set_shape(strata_batch *b, int rows, int blocks_per_cu, bool graph)
@@ -359,11 +374,10 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
         }
         b->own_stream = true;
     }
-    for (hipStream_t &st : b->side)
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-            delete b;
-            return fail(STRATA_EDEVICE, "strata: cannot create a stream");
-        }
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(STRATA_EDEVICE, "strata: cannot create a stream");
+    }
     for (hipEvent_t &e : b->ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
             delete b;

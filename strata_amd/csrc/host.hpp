@@ -35,12 +35,13 @@ struct ListLayout {
 size_t list_bytes(int nops, size_t ntasks);
 // Writes the lists of n tasks (made by make_task, in work-list order;
 // masked[i]: task i's output has a mask) into out, for `rows` rows per
-// radius-1 unit.
+// radius-1 unit, or (rows 0) rows chosen per task for latency.
 ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, const bool *masked, size_t n,
                        int rows, char *out);
-// The launch of lists uploaded to dev, with at most max_blocks[f]
-// workgroups per list.
-BatchLaunch launch_view(const ListLayout &L, char *dev, const int *max_blocks);
+// The launch of lists uploaded to dev, with at most the given workgroups
+// for the wave kernel (sweeps and validity) and the focal kernel; clear:
+// zero the counters with a kernel first (replays of uploaded lists).
+BatchLaunch launch_view(const ListLayout &L, char *dev, int max_wave_blocks, int max_focal_blocks, bool clear);
 
 } // namespace strata
 

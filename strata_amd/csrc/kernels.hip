@@ -957,77 +957,89 @@ constexpr int kCounterStride = 64;  // unsigned words
 
 __global__ __launch_bounds__(64) void batch_clear(unsigned *counters)
 {
-    if (threadIdx.x < kBatchFamilies) counters[threadIdx.x * kCounterStride] = 0u;
+    if (threadIdx.x < 2) counters[threadIdx.x * kCounterStride] = 0u;
 }
 
-// Sweep and validity units belong to one wave: each wave claims its own
-// (lane 0's atomic, broadcast with readfirstlane) and never waits for the
-// rest of its workgroup. Focal units use the workgroup's LDS tile: the
-// workgroup claims them together.
-// The lists and ops are read-only for the whole launch and never alias the
-// rasters the units write (__restrict__), so their fields stay in registers
-// across the units' stores instead of being reloaded after each one.
-template <int FAM>
-__global__ __launch_bounds__(256) void batch_kernel(const BatchHeader *__restrict__ hdr,
-                                                    const BatchTask *__restrict__ tasks,
-                                                    const BatchOp *__restrict__ ops, unsigned *counter)
+// Sweep units, then validity units, claimed one wave at a time from one
+// counter: lane 0 claims, readfirstlane broadcasts, and a wave never waits
+// for the rest of its workgroup. The lists and ops are read-only for the
+// whole launch and never alias the rasters the units write (__restrict__),
+// so their fields stay in registers across the units' stores instead of
+// being reloaded after each one.
+__global__ __launch_bounds__(256) void batch_wave_kernel(const BatchHeader *__restrict__ hdr,
+                                                         const BatchTask *__restrict__ sweeps,
+                                                         const BatchTask *__restrict__ valids,
+                                                         const BatchOp *__restrict__ ops, unsigned *counter)
 {
-    const int64_t ntasks = hdr->ntasks, nunits = hdr->nunits;
-    int64_t ti = 0, tu0 = 0, tu1 = 0;  // the current task and its units [tu0, tu1)
-    if constexpr (FAM == BF_FOCAL) {
-        __shared__ float lds[kBatchLdsFloats];
-        __shared__ unsigned claimed;
-        if (threadIdx.x == 0) claimed = atomicAdd(counter, 1u);
+    const int64_t nh = hdr[BF_HORN].nunits, nv = hdr[BF_VALIDITY].nunits;
+    const int64_t th = hdr[BF_HORN].ntasks, tv = hdr[BF_VALIDITY].ntasks;
+    const bool lead = (threadIdx.x & 63) == 0;
+    int64_t ti = 0, tu0 = 0, tu1 = 0;  // the current task and its units [tu0, tu1) in its list
+    bool in_valid = false;            // which list the cached task belongs to
+    unsigned c = 0u;
+    if (lead) c = atomicAdd(counter, 1u);
+    int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(c);
+    while (u < nh + nv) {
+        unsigned next = 0u;
+        if (lead) next = atomicAdd(counter, 1u);  // consumed after this unit
+        const bool valid = u >= nh;
+        const BatchTask *list = valid ? valids : sweeps;
+        const int64_t lu_all = valid ? u - nh : u;
+        if (valid != in_valid || lu_all < tu0 || lu_all >= tu1) {
+            ti = batch_find(list, valid ? tv : th, lu_all);
+            tu0 = list[ti].u0;
+            tu1 = list[ti].u1;
+            in_valid = valid;
+        }
+        const BatchTask &t = list[ti];
+        const BatchOp &op = ops[t.op];
+        const int64_t lu = lu_all - tu0;
+        if (!valid) {
+            switch (op.op) {
+            case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu); break;
+            case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu); break;
+            case STRATA_OP_HILLSHADE: batch_horn<STRATA_OP_HILLSHADE>(op, t, lu); break;
+            default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu); break;
+            }
+        } else {
+            batch_validity(op, t, lu);
+        }
+        u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(next);
+    }
+}
+
+// Focal units use the workgroup's LDS tile: the workgroup claims them
+// together.
+__global__ __launch_bounds__(256) void batch_focal_kernel(const BatchHeader *__restrict__ hdr,
+                                                          const BatchTask *__restrict__ tasks,
+                                                          const BatchOp *__restrict__ ops, unsigned *counter)
+{
+    __shared__ float lds[kBatchLdsFloats];
+    __shared__ unsigned claimed;
+    const int64_t ntasks = hdr[BF_FOCAL].ntasks, nunits = hdr[BF_FOCAL].nunits;
+    int64_t ti = 0, tu0 = 0, tu1 = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(counter, 1u);
+    __syncthreads();
+    int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+    while (u < nunits) {
+        unsigned next = 0u;
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);  // consumed after this unit
+        if (u < tu0 || u >= tu1) {
+            ti = batch_find(tasks, ntasks, u);
+            tu0 = tasks[ti].u0;
+            tu1 = tasks[ti].u1;
+        }
+        const BatchTask &t = tasks[ti];
+        const BatchOp &op = ops[t.op];
+        switch (op.radius) {
+        case 1: batch_focal<1>(op, t, u - tu0, lds); break;
+        case 2: batch_focal<2>(op, t, u - tu0, lds); break;
+        default: batch_focal<3>(op, t, u - tu0, lds); break;
+        }
+        __syncthreads();  // the unit is done (and the LDS free) on every wave
+        if (threadIdx.x == 0) claimed = next;
         __syncthreads();
-        int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
-        while (u < nunits) {
-            unsigned next = 0u;
-            if (threadIdx.x == 0) next = atomicAdd(counter, 1u);  // consumed after this unit
-            if (u < tu0 || u >= tu1) {
-                ti = batch_find(tasks, ntasks, u);
-                tu0 = tasks[ti].u0;
-                tu1 = tasks[ti].u1;
-            }
-            const BatchTask &t = tasks[ti];
-            const BatchOp &op = ops[t.op];
-            switch (op.radius) {
-            case 1: batch_focal<1>(op, t, u - tu0, lds); break;
-            case 2: batch_focal<2>(op, t, u - tu0, lds); break;
-            default: batch_focal<3>(op, t, u - tu0, lds); break;
-            }
-            __syncthreads();  // the unit is done (and the LDS free) on every wave
-            if (threadIdx.x == 0) claimed = next;
-            __syncthreads();
-            u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
-        }
-    } else {
-        const bool lead = (threadIdx.x & 63) == 0;
-        unsigned c = 0u;
-        if (lead) c = atomicAdd(counter, 1u);
-        int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(c);
-        while (u < nunits) {
-            unsigned next = 0u;
-            if (lead) next = atomicAdd(counter, 1u);  // consumed after this unit
-            if (u < tu0 || u >= tu1) {
-                ti = batch_find(tasks, ntasks, u);
-                tu0 = tasks[ti].u0;
-                tu1 = tasks[ti].u1;
-            }
-            const BatchTask &t = tasks[ti];
-            const BatchOp &op = ops[t.op];
-            const int64_t lu = u - tu0;
-            if constexpr (FAM == BF_HORN) {
-                switch (op.op) {
-                case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu); break;
-                case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu); break;
-                case STRATA_OP_HILLSHADE: batch_horn<STRATA_OP_HILLSHADE>(op, t, lu); break;
-                default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu); break;
-                }
-            } else {
-                batch_validity(op, t, lu);
-            }
-            u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(next);
-        }
+        u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
     }
 }
 
@@ -1207,37 +1219,31 @@ void batch_units(const BatchOp &op, BatchTask &t, int rows)
     t.vwaves_x = (uint32_t)cdiv(t.rg.w / 32 + 2, kVsOut);
 }
 
-hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side0, hipStream_t side1,
-                        hipEvent_t *ev)
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side, hipEvent_t *ev)
 {
-    hipLaunchKernelGGL(batch_clear, dim3(1), dim3(64), 0, st, L.counters);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const bool fork = side0 && side1 && ev;
-    if (fork) {
-        // The focal and validity lists run on two side streams, joined back
-        // before the batch counts as done (also inside a graph capture).
-        if ((e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side0, ev[0], 0)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side1, ev[0], 0)) != hipSuccess) return e;
+    hipError_t e;
+    if (L.clear) {
+        hipLaunchKernelGGL(batch_clear, dim3(1), dim3(64), 0, st, L.counters);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipStream_t on[kBatchFamilies] = {st, fork ? side0 : st, fork ? side1 : st};
-    if (L.nunits[BF_HORN] > 0)
-        hipLaunchKernelGGL(batch_kernel<BF_HORN>, dim3((unsigned)L.blocks[BF_HORN]), dim3(256), 0, on[BF_HORN],
-                           L.hdr[BF_HORN], L.tasks[BF_HORN], L.ops, L.counters + BF_HORN * kCounterStride);
-    if (L.nunits[BF_FOCAL] > 0)
-        hipLaunchKernelGGL(batch_kernel<BF_FOCAL>, dim3((unsigned)L.blocks[BF_FOCAL]), dim3(256), 0, on[BF_FOCAL],
-                           L.hdr[BF_FOCAL], L.tasks[BF_FOCAL], L.ops, L.counters + BF_FOCAL * kCounterStride);
-    if (L.nunits[BF_VALIDITY] > 0)
-        hipLaunchKernelGGL(batch_kernel<BF_VALIDITY>, dim3((unsigned)L.blocks[BF_VALIDITY]), dim3(256), 0,
-                           on[BF_VALIDITY], L.hdr[BF_VALIDITY], L.tasks[BF_VALIDITY], L.ops,
-                           L.counters + BF_VALIDITY * kCounterStride);
+    const bool focal = L.nunits[BF_FOCAL] > 0;
+    const bool fork = focal && side && ev;
+    if (fork) {
+        // The focal list runs on a side stream, joined back before the
+        // batch counts as done (also inside a graph capture).
+        if ((e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side, ev[0], 0)) != hipSuccess) return e;
+    }
+    if (focal)
+        hipLaunchKernelGGL(batch_focal_kernel, dim3((unsigned)L.blocks_focal), dim3(256), 0, fork ? side : st,
+                           L.hdr, L.tasks[BF_FOCAL], L.ops, L.counters + kCounterStride);
+    if (L.nunits[BF_HORN] + L.nunits[BF_VALIDITY] > 0)
+        hipLaunchKernelGGL(batch_wave_kernel, dim3((unsigned)L.blocks_wave), dim3(256), 0, st, L.hdr,
+                           L.tasks[BF_HORN], L.tasks[BF_VALIDITY], L.ops, L.counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (fork) {
-        if ((e = hipEventRecord(ev[1], side0)) != hipSuccess) return e;
-        if ((e = hipEventRecord(ev[2], side1)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev[1], side)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(st, ev[1], 0)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(st, ev[2], 0)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -1245,12 +1251,9 @@ hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side0,
 int batch_blocks_per_cu(int family)
 {
     int n = 0;
-    hipError_t e = hipErrorInvalidValue;
-    switch (family) {
-    case BF_HORN: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_HORN>, 256, 0); break;
-    case BF_FOCAL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_FOCAL>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<BF_VALIDITY>, 256, 0); break;
-    }
+    const hipError_t e =
+        family == BF_FOCAL ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_focal_kernel, 256, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_wave_kernel, 256, 0);
     return (e != hipSuccess || n < 1) ? 1 : n;
 }
 

@@ -125,7 +125,9 @@ struct BatchTask {
     int64_t u0, u1;
 };
 
-// Work lists of a batch, one persistent kernel each.
+// Work lists of a batch. The radius-1 sweep list and the validity list are
+// run by one persistent kernel (sweep units first), the focal list by a
+// second one beside it.
 enum BatchFamily : int {
     BF_HORN = 0,      // radius-1 terrain sweeps
     BF_FOCAL = 1,     // focal tiles
@@ -137,27 +139,26 @@ struct BatchHeader {
     int64_t ntasks, nunits;
 };
 
-// Device pointers of one batch's lists (headers, tasks sorted by u0, one
-// claim counter each) and the launch shape.
+// Device pointers of one batch's lists (the three headers, each list's
+// tasks sorted by u0, two claim counters 256 bytes apart: sweeps and
+// validity, then focal) and the launch shape.
 struct BatchLaunch {
     const BatchOp *ops;
-    const BatchHeader *hdr[kBatchFamilies];
+    const BatchHeader *hdr;  // kBatchFamilies headers
     const BatchTask *tasks[kBatchFamilies];
-    unsigned *counters;  // kBatchFamilies counters
+    unsigned *counters;
     int64_t nunits[kBatchFamilies];
-    int blocks[kBatchFamilies];
+    int blocks_wave, blocks_focal;
+    bool clear;  // zero the counters first (they are not uploaded zeroed)
 };
 
 // Fills a task's unit shape for `rows` rows per radius-1 unit: rows,
 // blocks_x (data units across) and vwaves_x.
 void batch_units(const BatchOp &op, BatchTask &t, int rows);
 
-// Clears the claim counters (a kernel, so it can live in a captured graph)
-// and runs the lists' persistent kernels: on st alone, or, given two side
-// streams and three events, the focal and validity lists on the side
-// streams joined back into st.
-hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side0, hipStream_t side1,
-                        hipEvent_t *ev);
+// Runs the lists' persistent kernels on st; with a side stream and two
+// events, the focal list runs on the side stream, joined back into st.
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side, hipEvent_t *ev);
 
 // Workgroups of a list's kernel resident per CU (occupancy), for sizing.
 int batch_blocks_per_cu(int family);

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -80,7 +81,7 @@ enum ReqState { PENDING = 0, RUNNING = 1, DONE = 2, RELEASED = 3 };
 
 struct Request {
     strata_window w;
-    double t_arrival = 0, t_done = 0;
+    double t_arrival = 0, t_launch = 0, t_done = 0;
     int state = PENDING;
     bool drop = false;  // release the result as soon as it completes
     size_t off = 0, bytes = 0, mask_off = 0;
@@ -114,6 +115,7 @@ struct strata_server {
     std::vector<Request> reqs;
     std::thread sched;
     bool stop = false;
+    std::atomic<uint64_t> gen{0};  // bumped (under mu) by submit, release and stop
     int64_t in_flight = 0;
     strata_serve_stats stats{};
     double busy_since = -1;
@@ -143,6 +145,8 @@ void strata_server::complete(Slot &s, double now)
         r.t_done = now;
         r.state = DONE;
         stats.requests += 1;
+        stats.queue_ms += (r.t_launch - r.t_arrival) * 1e3;
+        stats.service_ms += (r.t_done - r.t_launch) * 1e3;
         stats.cells += r.w.w * r.w.h;
         if (r.drop) {
             arena.release(r.off, r.bytes);
@@ -192,13 +196,16 @@ bool strata_server::launch(Slot &s, std::vector<int64_t> &batch, std::unique_loc
     in_flight += (int64_t)batch.size();
     stats.max_in_flight = std::max(stats.max_in_flight, in_flight);
     stats.batches += 1;
-    const BatchLaunch view = launch_view(L, s.dev, max_blocks);
+    // The upload carries zeroed counters: no clearing kernel.
+    const BatchLaunch view = launch_view(L, s.dev, max_blocks[BF_HORN], max_blocks[BF_FOCAL], false);
 
     lk.unlock();
     hipError_t e = hipMemcpyAsync(s.dev, s.host, L.bytes, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess) e = launch_batch(view, s.stream, nullptr, nullptr, nullptr);
+    if (e == hipSuccess) e = launch_batch(view, s.stream, nullptr, nullptr);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    const double launched = seconds(Clock::now());
     lk.lock();
+    for (int64_t id : s.reqs) reqs[(size_t)id].t_launch = launched;
     if (e != hipSuccess) {
         sched_error = hip_fail(e, "launching a served batch");
         return false;
@@ -257,12 +264,25 @@ void strata_server::loop()
             lk.unlock();
             std::this_thread::yield();
             lk.lock();
-        } else if (blocked) {
-            // The queue head waits for result memory that only a release
-            // (or a new submission, or stop) can change.
-            cv.wait(lk);
         } else {
-            cv.wait(lk, [&] { return stop || (!pending.empty() && sched_error == STRATA_OK); });
+            // Nothing runs: the queue is empty, or its head waits for result
+            // memory. Only a submission, a release or stop changes that. Spin
+            // on their generation count for a while (a request arriving soon
+            // is launched without a thread wake-up), then sleep.
+            (void)blocked;
+            const uint64_t g = gen.load();
+            lk.unlock();
+            const Clock::time_point until = Clock::now() + std::chrono::milliseconds(2);
+            bool changed = false;
+            while (Clock::now() < until) {
+                if (gen.load() != g) {
+                    changed = true;
+                    break;
+                }
+                std::this_thread::yield();
+            }
+            lk.lock();
+            if (!changed) cv.wait(lk, [&] { return stop || gen.load() != g; });
         }
     }
 }
@@ -287,7 +307,7 @@ int strata_serve_create(const strata_serve_config *cfg, strata_server **out)
     s->cfg = *cfg;
     if (s->cfg.slots <= 0) s->cfg.slots = 4;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
-    if (s->cfg.rows <= 0) s->cfg.rows = 64;
+    if (s->cfg.rows < 0) s->cfg.rows = 0;  // 0: rows chosen per request for latency
     if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)4 << 30;
     auto keep = [&](strata_kernel &k) {
         if (k.ntaps > 0) {
@@ -351,6 +371,7 @@ static int submit_at(strata_server *s, const strata_window *w, double t_arrival,
     s->reqs.push_back(r);
     s->pending.push_back(n);
     if (id) *id = n;
+    s->gen.fetch_add(1);
     s->cv.notify_all();
     return STRATA_OK;
 }
@@ -400,6 +421,7 @@ int strata_serve_release(strata_server *s, int64_t id)
     Request &r = s->reqs[(size_t)id];
     s->arena.release(r.off, r.bytes);
     r.state = RELEASED;
+    s->gen.fetch_add(1);
     s->cv.notify_all();
     return STRATA_OK;
 }
@@ -450,6 +472,7 @@ void strata_serve_destroy(strata_server *s)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        s->gen.fetch_add(1);
     }
     s->cv.notify_all();
     if (s->sched.joinable()) s->sched.join();
