@@ -24,9 +24,13 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -66,6 +70,9 @@ def parse():
                          "small request's rows in parallel)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the Slope kernel's HBM traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -83,6 +90,9 @@ def main() -> int:
     from strata_amd import _lib, synth, terrain
     from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked, Stats
     from strata_amd.raster import Float32Raster
+
+    if args.pmc_child:
+        return pmc_child(args)
 
     dev = torch.device("cuda:0")
     L = _lib.lib()
@@ -178,6 +188,10 @@ def main() -> int:
             log(f"bench: {name} {suite[name]}")
         del out2
 
+    traffic = None
+    if not args.no_traffic:
+        traffic = pmc_traffic(args, N, log)
+
     batch = None
     if args.batch_tiles > 0:
         del out, sink
@@ -214,7 +228,9 @@ def main() -> int:
                    "stats": {"cells": st.Cells, "tiles": st.Tiles, "bands": st.Bands,
                              "amplification": round(st.Amplification(), 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic and "bytes_per_launch" in traffic else None,
+                     "traffic_detail": traffic,
                      "kernel": "horn_sweep<SLOPE> (data kernel)", "bytes_per_launch": data_bytes,
                      "kernel_ms": round(kern_ms, 4), "launches": timing.data_launches},
         "op_roofline": {"bytes_per_step": op_bytes, "frac": round(op_bytes / (el / args.steps) / 1e9 /
@@ -232,6 +248,91 @@ def main() -> int:
                                    "arrival to completion"}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def pmc_child(args) -> int:
+    """The workload of one counter pass: the Slope op twice on the same DEM,
+    then one device copy of the DEM (the calibration kernel)."""
+    import torch
+
+    from strata_amd import _lib, synth, terrain
+    from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked
+    from strata_amd.raster import Float32Raster
+
+    dev = torch.device("cuda:0")
+    L = _lib.lib()
+    assert L.strata_init(0) == 0, _lib.last_error()
+    if args.sweep_rows:
+        t = _lib.Tuning()
+        L.strata_get_tuning(t)
+        t.sweep_rows = args.sweep_rows
+        assert L.strata_set_tuning(t) == 0, _lib.last_error()
+    N = args.size
+    data = synth.fbm_dem(N, N, hurst=0.8, seed=1, lo=0.0, hi=3000.0, device=dev)
+    valid, _ = synth.blob_nodata(N, N, frac=0.005, rmin=5, rmax=50, seed=2, device=dev)
+    dem = Float32Raster(data, N, N, N, valid, 0)
+    out = Float32Raster(torch.empty(N * N, dtype=torch.float32, device=dev), N, N, N, torch.zeros_like(valid), 0)
+    slope = terrain.slopeOp(terrain.SlopeOptions(CellSize=30, Units=terrain.SlopeDegrees))
+    for _ in range(2):
+        assert ProcessChunked(None, [MemorySink(out)], [MemorySource(dem)], slope,
+                              Options(TileHeight=args.tile_height)) is None
+    tmp = torch.empty_like(data)
+    tmp.copy_(data)
+    torch.cuda.synchronize()
+    return 0
+
+
+def pmc_traffic(args, N, log):
+    """HBM bytes per launch of the Slope data kernel from two rocprofv3
+    counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+    scaled by what the same counter reads for a plain device copy of the
+    DEM (4·N² bytes read and written) in the same run."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return {"error": "rocprofv3 not found"}
+    copy_kib = 4.0 * N * N / 1024.0
+    got = {}
+    with tempfile.TemporaryDirectory(prefix="strata_pmc_") as d:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(d, ctr)
+            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--pmc-child", "--size", str(N),
+                   "--tile-height", str(args.tile_height)]
+            if args.sweep_rows:
+                cmd += ["--sweep-rows", str(args.sweep_rows)]
+            log(f"bench: counter pass {ctr}")
+            try:
+                p = subprocess.run(cmd, capture_output=True, text=True, timeout=150)
+            except subprocess.TimeoutExpired:
+                return {"error": f"rocprofv3 --pmc {ctr} timed out"}
+            files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+            if p.returncode != 0 or not files:
+                return {"error": f"rocprofv3 --pmc {ctr} failed ({p.returncode})"}
+            import csv
+            slope_vals, copy_vals = [], []
+            with open(files[0]) as f:
+                for row in csv.DictReader(f):
+                    name = row.get("Kernel_Name", "")
+                    v = float(row["Counter_Value"])
+                    if "horn_sweep<2>" in name:
+                        slope_vals.append(v)
+                    elif "copyBuffer" in name:
+                        copy_vals.append(v)
+            if not slope_vals or not copy_vals:
+                return {"error": f"rocprofv3 --pmc {ctr}: kernels not found"}
+            got[ctr] = (sum(slope_vals) / len(slope_vals), max(copy_vals))
+    ff = copy_kib / got["FETCH_SIZE"][1]
+    wf = copy_kib / got["WRITE_SIZE"][1]
+    fetch = got["FETCH_SIZE"][0] * ff * 1024
+    write = got["WRITE_SIZE"][0] * wf * 1024
+    res = {"bytes_per_launch": round(fetch + write), "read_bytes": round(fetch), "write_bytes": round(write),
+           "fetch_kib_raw": round(got["FETCH_SIZE"][0], 1), "write_kib_raw": round(got["WRITE_SIZE"][0], 1),
+           "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+           "calibration": f"device copy of the {N}x{N} DEM ({copy_kib:.0f} KiB read and written), same run",
+           "algorithmic_bytes": 8 * N * N}
+    log(f"bench: traffic {res['bytes_per_launch'] / 1e9:.3f} GB/launch (read {fetch / 1e9:.3f}, write "
+        f"{write / 1e9:.3f}; factors {ff:.3f}/{wf:.3f})")
+    return res
 
 
 def batch_tiles(n, N, seed):
