@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--serve-rows", type=int, default=0,
                     help="rows per sweep unit when serving (0: chosen per request; short units run a "
                          "small request's rows in parallel)")
+    ap.add_argument("--focal-size", type=int, default=16384,
+                    help="side of the focal-suite raster (0: skip the focal suite)")
+    ap.add_argument("--focal-steps", type=int, default=5, help="timed runs of each focal op")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -198,6 +201,10 @@ def main() -> int:
         torch.cuda.empty_cache()
         batch = batch_bench(args, dem, N, dev, log)
 
+    focal_res = None
+    if args.focal_size > 0:
+        focal_res = focal_bench(args, dev, log)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serving_bench(args, dem, N, log)
@@ -240,6 +247,7 @@ def main() -> int:
         "suite": suite,
         "batch": batch,
         "serving": serving,
+        "focal": focal_res,
     }
     if serving and serving["levels"]:
         at = min(serving["levels"], key=lambda lv: abs(lv["qps"] - 1000.0))
@@ -421,6 +429,81 @@ def batch_bench(args, dem, N, dev, log):
         f"({res['batched']['gcells_s']} Gcells/s, shape {shape}), per tile {per_tile_s * 1e3:.2f} ms, "
         f"speedup {res['speedup']}x, identical {identical}")
     del out, mask
+    torch.cuda.empty_cache()
+    return res
+
+
+def focal_bench(args, dev, log):
+    """Focal ops over an S×S raster of seeded uniform values in [0, 1000)
+    with 2% i.i.d. NoData: Correlate and Convolve at r = 1, 2, 4, 8 with
+    seeded weights U[-1, 1], CorrelateSeparable with Gaussian taps
+    (sigma = r/2) at r = 1, 2, 4, 8, and Mean, Min, Max at r = 3; plain
+    ProcessN over the device-resident raster."""
+    import numpy as np
+    import torch
+
+    from strata_amd import _lib, focal, synth
+    from strata_amd.engine import Options, ProcessN
+    from strata_amd.raster import Float32Raster
+
+    S = args.focal_size
+    cells = S * S
+    data = synth.uniform_raster(cells, 0.0, 1000.0, seed=3, device=dev)
+    valid = synth.random_nodata(cells, 0.02, seed=4, device=dev)
+    src = Float32Raster(data, S, S, S, valid, 0)
+    dst = Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), S, S, S, torch.zeros_like(valid), 0)
+    rng = np.random.default_rng(5)
+    ops = []
+    for r in (1, 2, 4, 8):
+        n = 2 * r + 1
+        w = rng.uniform(-1, 1, n * n).astype(np.float32)
+        ops.append((f"Correlate r={r}", focal.newWeightsKernel(focal.WeightsOptions(r, w), False), 2 * n * n - 1))
+        ops.append((f"Convolve r={r}", focal.newWeightsKernel(focal.WeightsOptions(r, w), True), 2 * n * n - 1))
+    errs = {}
+    for r in (4, 8):
+        n = 2 * r + 1
+        w = np.random.default_rng(50 + r).uniform(-1, 1, n * n).astype(np.float32)
+        k32 = focal.newWeightsKernel(focal.WeightsOptions(r, w), False)
+        kmf = focal.newWeightsKernel(focal.WeightsOptions(r, w), False, mfma=True)
+        ops.append((f"Correlate r={r} MFMA split-BF16x3", kmf, 2 * n * n - 1))
+        errs[r] = (k32, kmf)
+    for r in (1, 2, 4, 8):
+        g = focal.Gaussian(r, r / 2)
+        ops.append((f"CorrelateSeparable Gaussian r={r}", focal.newSeparableKernel(focal.SeparableOptions(r, g, g)),
+                    2 * 2 * (2 * r + 1) - 2))
+    for op in ("mean", "min", "max"):
+        ops.append((f"{op.capitalize()} r=3", focal.newBox(focal.BoxOptions(3), op), 13 if op == "mean" else 12))
+    res = {"workload": f"{S}x{S} float32 uniform [0,1000) seeded, 2% i.i.d. NoData; plain ProcessN, "
+                       "device-resident", "ops": {}}
+    for name, k, flops in ops:
+        assert ProcessN(None, [dst], [src], k, Options()) is None  # warm-up
+        torch.cuda.synchronize()
+        timing = _lib.KernelTiming().begin()
+        t = time.perf_counter()
+        for _ in range(args.focal_steps):
+            assert ProcessN(None, [dst], [src], k, Options(), async_=True) is None
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t) / args.focal_steps
+        timing.end()
+        kms = timing.data_ms / max(1, timing.data_launches)
+        res["ops"][name] = {"gcells_s": round(cells / el / 1e9, 2), "ms": round(el * 1e3, 4),
+                            "data_kernel_ms": round(kms, 4),
+                            "hbm_frac": round(8.25 * cells / el / 1e9 / HBM_PEAK_GBS, 4),
+                            "fp32_tflops": round(flops * cells / (kms * 1e-3) / 1e12, 2)}
+        log(f"bench: focal {name}: {res['ops'][name]}")
+    # Precision of both Correlate paths against the FP64 correlation (on the
+    # device), over every interior cell.
+    res["error_vs_f64"] = {}
+    for r, (k32, kmf) in errs.items():
+        for label, k in (("fp32", k32), ("mfma_bf16x3", kmf)):
+            assert ProcessN(None, [dst], [src], k, Options()) is None
+            torch.cuda.synchronize()
+            e = focal.CorrelateError(k, src, dst)
+            res["error_vs_f64"][f"r={r} {label}"] = {"max_abs": e["max_abs"], "max_rel": e["max_rel"],
+                                                     "max_abs_over_sum_abs_wx": e["max_scaled"],
+                                                     "cells": e["cells"]}
+            log(f"bench: focal error r={r} {label}: {res['error_vs_f64'][f'r={r} {label}']}")
+    del src, dst, data, valid
     torch.cuda.empty_cache()
     return res
 

@@ -88,8 +88,10 @@ enum strata_op {
 #define STRATA_FLAG_FIT 0x100  /* terrain op through Wood's quadratic fit,    */
                                /* radius = FitRadius (terrain/fit.go:187-226) */
 #define STRATA_CURV_SHIFT 4    /* curvature kind (0 profile, 1 plan, 2 mean)  */
-#define STRATA_FLAG_MFMA 0x200 /* correlate r>=4: split-BF16x3 MFMA path      */
-                               /* (approximate; see strata_process docs)     */
+#define STRATA_FLAG_MFMA 0x200 /* Correlate: split-BF16x3 MFMA path (meant for */
+                               /* r >= 4). Approximate: each product within    */
+                               /* about 3*2^-18 of |w*x|; windows holding a    */
+                               /* non-finite value get the exact FP32 result.  */
 
 /*
  * k[] layout by op (all already-resolved float32 constants):
@@ -254,6 +256,15 @@ int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, ui
 /* ------------------------------------------------------------------ */
 /* Measurement (no counterpart in the reference, which has only Stats)  */
 /* ------------------------------------------------------------------ */
+
+/* Error of a Correlate output against the FP64 correlation of src (products
+ * and sums in double, on the device) over the interior cells whose window
+ * is finite: max |out - ref|, max |out - ref| / |ref| and max
+ * |out - ref| / sum |w x|, and the number of cells compared. Used to report
+ * the precision of the FP32 path and of the split-BF16x3 MFMA path
+ * (STRATA_FLAG_MFMA). Device rasters only. */
+int strata_correlate_error(const strata_kernel *k, const strata_raster *src, const strata_raster *out,
+                           double *max_abs, double *max_rel, double *max_scaled, int64_t *cells);
 
 /* Start recording HIP events around every data and validity kernel launch
  * (on the launch's own stream), discarding earlier records. */

@@ -210,6 +210,7 @@ EXPORTS = (
     "strata_selfcheck",
     "strata_timing_begin",
     "strata_timing_end",
+    "strata_correlate_error",
     "strata_batch_create",
     "strata_batch_run",
     "strata_batch_run_per_tile",
@@ -281,6 +282,8 @@ def _bind_optional(L) -> None:
 
 
 _OPTIONAL: list = [
+    ("strata_correlate_error", [POINTER(Kernel), POINTER(Raster), POINTER(Raster), POINTER(ctypes.c_double),
+                                POINTER(ctypes.c_double), POINTER(ctypes.c_double), POINTER(c_int64)], c_int),
     ("strata_batch_create", [POINTER(Task), c_int, POINTER(BatchOptions), POINTER(c_void_p)], c_int),
     ("strata_batch_run", [c_void_p], c_int),
     ("strata_batch_run_per_tile", [c_void_p], c_int),

@@ -621,6 +621,210 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
 }
 
 // ---------------------------------------------------------------------------
+// correlate_mfma: focal Correlate/Convolve as split-BF16x3 matrix products
+// ---------------------------------------------------------------------------
+//
+// For a strip of 16 output columns at x0, weight row i contributes
+// out[y][x0+n] += sum_k X_i[y][k] * T_i[k][n], with X_i[y][k] the input at
+// row y+i-r, column x0-r+k, and T_i[k][n] = w[i][k-n] for 0 <= k-n <= 2r
+// (else 0): a (rows x 32) by (32 x 16) product, since 16+2r <= 32. Each
+// operand is split into two bf16 parts, x = hi + lo (hi = bf16(x), lo =
+// bf16(x - hi)), and each 16x16 output block accumulates hi*hi + hi*lo +
+// lo*hi over the 2r+1 weight rows with v_mfma_f32_16x16x32_bf16 in fp32.
+// The dropped lo*lo term and the two bf16 roundings bound each product's
+// error by about 3*2^-18 of |w*x|: this path is approximate, unlike the
+// FP32 path, which follows the reference's operation order exactly.
+//
+// A workgroup computes a 64x64 tile: wave w takes columns 16w..16w+15 and
+// four 16-row blocks. The input tile and halo are staged in LDS as the hi
+// and lo planes (rows padded so each lane's 8 operand values are one
+// aligned 16-byte read). Operand layouts of the 16x16x32 bf16 MFMA: lane l
+// holds A[m = l%16][k = 8(l/16) .. +7], B[k = 8(l/16) .. +7][n = l%16], and
+// C[m = 4(l/16) .. +3][n = l%16].
+//
+// Non-finite inputs cannot be split (Inf - Inf is NaN) and would leak into
+// neighbouring outputs through the zeros of T_i, so they are staged as 0
+// and every output whose window holds one is recomputed in the exact FP32
+// order, which gives the FP32 path's value there.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaTile = 64;
+
+// v = h + l with h = bf16(v) and l = bf16(v - h), for finite v. Finite
+// values above the largest bf16 (0x7f7f) would round to infinity; their
+// high part is truncated instead, which keeps v - h finite.
+__device__ __forceinline__ void split_bf16(float v, __bf16 &h, __bf16 &l)
+{
+    __bf16 t = (__bf16)v;
+    if (__builtin_isinf((float)t)) t = __builtin_bit_cast(__bf16, (uint16_t)(f2u(v) >> 16));
+    h = t;
+    l = (__bf16)(v - (float)t);
+}
+
+template <int R>
+constexpr int mfma_pitch()  // bf16 per staged row: at least 48 + 32 columns, a multiple of 8
+{
+    return ((kMfmaTile + 2 * R + 7) / 8) * 8 + 8;
+}
+
+// Exact FP32 correlate of one cell, in the reference's order; also reports
+// whether the window holds a non-finite value.
+template <int R>
+__device__ __forceinline__ float correlate_cell(const float *taps, const SrcView &s, int64_t x, int64_t y,
+                                                bool &nonfinite)
+{
+    constexpr int N = 2 * R + 1;
+    float acc = 0.0f;
+    nonfinite = false;
+    for (int j = 0; j < N; ++j)
+        for (int c = 0; c < N; ++c) {
+            const float v = load_cell(s, x - R + c, y - R + j);
+            nonfinite = nonfinite || !__builtin_isfinite(v);
+            const float term = taps[j * N + c] * v;
+            acc = (j == 0 && c == 0) ? term : acc + term;
+        }
+    return acc;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg, SrcView s, DstView d0)
+{
+    constexpr int N = 2 * R + 1;
+    constexpr int LH = kMfmaTile + 2 * R, LW = kMfmaTile + 2 * R, PITCH = mfma_pitch<R>();
+    __shared__ __attribute__((aligned(16))) __bf16 hi[LH * PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 lo[LH * PITCH];
+    __shared__ float wts[N * N];
+    __shared__ int nonfinite;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kMfmaTile;
+    const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kMfmaTile;
+    if (tid == 0) nonfinite = 0;
+    for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
+    __syncthreads();
+    bool any = false;
+    for (int i = tid; i < LH * PITCH; i += 256) {
+        const int row = i / PITCH, col = i - row * PITCH;
+        float v = col < LW ? load_cell(s, tx0 - R + col, ty0 - R + row) : 0.0f;
+        if (!__builtin_isfinite(v)) {
+            any = true;
+            v = 0.0f;
+        }
+        split_bf16(v, hi[i], lo[i]);
+    }
+    if (any) nonfinite = 1;
+    __syncthreads();
+
+    const int n = lane & 15, kg = lane >> 4;
+    const int c0 = wave * 16;
+    f32x4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        bf16x8 bh, bl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int j = 8 * kg + e - n;
+            const float w = (j >= 0 && j < N) ? wts[i * N + j] : 0.0f;
+            __bf16 h, l;
+            split_bf16(w, h, l);
+            bh[e] = h;
+            bl[e] = l;
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int at = (16 * b + n + i) * PITCH + c0 + 8 * kg;  // A: row m = n, columns 8kg..8kg+7
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&hi[at]);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&lo[at]);
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[b], 0, 0, 0);
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[b], 0, 0, 0);
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[b], 0, 0, 0);
+        }
+    }
+
+    const int64_t x = tx0 + c0 + n;
+    const bool col_in = x < rg.ox + rg.w;
+    const bool col_interior = x >= R && x < rg.W - R;
+    const bool fix = nonfinite != 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t y = ty0 + 16 * b + 4 * kg + e;
+            if (!col_in || y >= rg.oy + rg.h) continue;
+            const bool interior = col_interior && y >= R && y < rg.H - R;
+            float v = acc[b][e];
+            if (fix && interior) {
+                bool nf;
+                const float exact = correlate_cell<R>(wts, s, x, y, nf);
+                if (nf) v = exact;
+            }
+            gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = interior ? v : edge_nan();
+        }
+}
+
+// ---------------------------------------------------------------------------
+// correlate_error_f64: a Correlate output against the FP64 correlation
+// ---------------------------------------------------------------------------
+//
+// Measurement only: for every interior cell of the region whose window is
+// finite, the FP64 correlation (products and sum in double) and the error
+// of out there: max |out - ref|, max |out - ref| / |ref| (ref != 0) and max
+// |out - ref| / sum |w x|; non-negative doubles order like their bit
+// patterns, so the maxima are unsigned 64-bit atomics of one value per
+// workgroup. acc: {max abs, max rel, max scaled, cells counted}.
+template <int R>
+__global__ __launch_bounds__(256) void correlate_error_f64(const Params P, Region rg, SrcView s, const float *out,
+                                                           int64_t out_stride, unsigned long long *acc)
+{
+    constexpr int N = 2 * R + 1;
+    __shared__ double red[3][256];
+    __shared__ unsigned long long cnt[256];
+    const int64_t x = rg.ox + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t y = rg.oy + blockIdx.y;
+    double ea = 0.0, er = 0.0, es = 0.0;
+    unsigned long long c = 0;
+    if (x < rg.ox + rg.w && x >= R && x < rg.W - R && y >= R && y < rg.H - R) {
+        double sum = 0.0, mag = 0.0;
+        bool finite = true;
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                const double v = (double)load_cell(s, x - R + k, y - R + j);
+                finite = finite && __builtin_isfinite(v);
+                const double t = (double)P.taps[j * N + k] * v;
+                sum += t;
+                mag += __builtin_fabs(t);
+            }
+        const double o = (double)gp(out)[(y - rg.oy) * out_stride + (x - rg.ox)];
+        if (finite) {
+            const double d = __builtin_fabs(o - sum);
+            ea = d;
+            er = sum != 0.0 ? d / __builtin_fabs(sum) : (d == 0.0 ? 0.0 : d);
+            es = mag > 0.0 ? d / mag : d;
+            c = 1;
+        }
+    }
+    red[0][threadIdx.x] = ea;
+    red[1][threadIdx.x] = er;
+    red[2][threadIdx.x] = es;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = max(red[q][threadIdx.x], red[q][threadIdx.x + st]);
+            cnt[threadIdx.x] += cnt[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 3; ++q) atomicMax(acc + q, (unsigned long long)__builtin_bit_cast(uint64_t, red[q][0]));
+        atomicAdd(acc + 3, cnt[0]);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
 // point_rows: radius-0 transfer kernels
 // ---------------------------------------------------------------------------
 
@@ -1131,6 +1335,69 @@ static hipError_t run_focal(const Params &P, const Region &rg, const SrcView &s,
     return hipErrorInvalidValue;
 }
 
+template <int R>
+static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s, const DstView *d, hipStream_t st)
+{
+    const int64_t gx = cdiv(rg.w, kMfmaTile);
+    for (int64_t yb = 0; yb < cdiv(rg.h, kMfmaTile); yb += kMaxGridY) {
+        Region sub = rg;
+        sub.oy = rg.oy + yb * kMfmaTile;
+        sub.h = min(rg.h - yb * kMfmaTile, kMaxGridY * kMfmaTile);
+        DstView e0 = d[0];
+        e0.p += yb * kMfmaTile * e0.stride;
+        hipLaunchKernelGGL(correlate_mfma<R>, dim3((unsigned)gx, (unsigned)cdiv(sub.h, kMfmaTile)), dim3(256), 0, st,
+                           P, sub, s, e0);
+    }
+    return hipGetLastError();
+}
+
+static hipError_t run_mfma(const Params &P, const Region &rg, const SrcView &s, const DstView *d, hipStream_t st)
+{
+    switch (P.radius) {
+    case 1: return run_mfma_r<1>(P, rg, s, d, st);
+    case 2: return run_mfma_r<2>(P, rg, s, d, st);
+    case 3: return run_mfma_r<3>(P, rg, s, d, st);
+    case 4: return run_mfma_r<4>(P, rg, s, d, st);
+    case 5: return run_mfma_r<5>(P, rg, s, d, st);
+    case 6: return run_mfma_r<6>(P, rg, s, d, st);
+    case 7: return run_mfma_r<7>(P, rg, s, d, st);
+    case 8: return run_mfma_r<8>(P, rg, s, d, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_correlate_error(const Params &P, const Region &rg, const SrcView &s, const float *out,
+                                  int64_t out_stride, unsigned long long *acc, hipStream_t st)
+{
+    if (rg.w <= 0 || rg.h <= 0) return hipSuccess;
+    const int64_t gx = cdiv(rg.w, 256);
+    for (int64_t yb = 0; yb < rg.h; yb += kMaxGridY) {
+        Region sub = rg;
+        sub.oy = rg.oy + yb;
+        sub.h = min(rg.h - yb, kMaxGridY);
+        const float *o = out + yb * out_stride;
+        const dim3 grid((unsigned)gx, (unsigned)sub.h);
+        switch (P.radius) {
+#define STRATA_ERR_CASE(RR)                                                                                   \
+    case RR:                                                                                                  \
+        hipLaunchKernelGGL(correlate_error_f64<RR>, grid, dim3(256), 0, st, P, sub, s, o, out_stride, acc);    \
+        break;
+            STRATA_ERR_CASE(1)
+            STRATA_ERR_CASE(2)
+            STRATA_ERR_CASE(3)
+            STRATA_ERR_CASE(4)
+            STRATA_ERR_CASE(5)
+            STRATA_ERR_CASE(6)
+            STRATA_ERR_CASE(7)
+            STRATA_ERR_CASE(8)
+#undef STRATA_ERR_CASE
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
 template <int OP>
 static hipError_t run_point(const Params &P, const Region &rg, const SrcView &s, const DstView *d, hipStream_t st)
 {
@@ -1160,7 +1427,9 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
     case STRATA_OP_ASPECT: return run_horn<STRATA_OP_ASPECT>(P, rg, s, d, nout, st);
     case STRATA_OP_HILLSHADE: return run_horn<STRATA_OP_HILLSHADE>(P, rg, s, d, nout, st);
     case STRATA_OP_CURVATURE: return run_horn<STRATA_OP_CURVATURE>(P, rg, s, d, nout, st);
-    case STRATA_OP_CORRELATE: return run_focal<FK_CORRELATE>(P, rg, s, d, nout, st);
+    case STRATA_OP_CORRELATE:
+        if (P.flags & STRATA_FLAG_MFMA) return run_mfma(P, rg, s, d, st);
+        return run_focal<FK_CORRELATE>(P, rg, s, d, nout, st);
     case STRATA_OP_SEPARABLE: return run_focal<FK_SEPARABLE>(P, rg, s, d, nout, st);
     case STRATA_OP_MEAN: return run_focal<FK_MEAN>(P, rg, s, d, nout, st);
     case STRATA_OP_MIN: return run_focal<FK_MIN>(P, rg, s, d, nout, st);

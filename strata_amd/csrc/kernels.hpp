@@ -73,6 +73,13 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
 hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has_src, const MaskView &dst,
                            hipStream_t st);
 
+// Error of a Correlate output (out, region-relative rows of out_stride)
+// against the FP64 correlation over the region's interior cells with a
+// finite window; acc (device, zeroed) receives {max abs, max rel, max
+// |err| / sum|w x|} as double bit patterns and the count of cells.
+hipError_t launch_correlate_error(const Params &P, const Region &rg, const SrcView &s, const float *out,
+                                  int64_t out_stride, unsigned long long *acc, hipStream_t st);
+
 // Self-check of a fast path against the full sequence over n inputs (see
 // selfcheck_kernel); dev3 holds {mismatches, first mismatch index, checked}.
 hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long long *dev3, hipStream_t st);

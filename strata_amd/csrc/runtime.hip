@@ -504,6 +504,40 @@ int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, ui
     return STRATA_OK;
 }
 
+int strata_correlate_error(const strata_kernel *k, const strata_raster *src, const strata_raster *out,
+                           double *max_abs, double *max_rel, double *max_scaled, int64_t *cells)
+{
+    int rc = check_kernel(k, 1, 1);
+    if (rc) return rc;
+    if (k->op != STRATA_OP_CORRELATE) return fail(STRATA_EINVAL, "strata: the error report is for Correlate kernels");
+    if (!src || !out) return fail(STRATA_EINVAL, "strata: nil raster");
+    if ((rc = check_raster("src", 0, *src)) || (rc = check_raster("dst", 0, *out))) return rc;
+    if (src->mem != STRATA_MEM_DEVICE || out->mem != STRATA_MEM_DEVICE)
+        return fail(STRATA_EINVAL, "strata: the error report reads device rasters");
+    if (src->width != out->width || src->height != out->height)
+        return fail(STRATA_EINVAL, "engine: src[0] is %lld×%lld, dst[0] is %lld×%lld", (long long)src->width,
+                    (long long)src->height, (long long)out->width, (long long)out->height);
+    const Params P = make_params(k);
+    const Region rg{src->width, src->height, 0, 0, src->width, src->height};
+    const SrcView sv = src_view(*src, 0, 0, src->width, src->height);
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, 4 * sizeof(unsigned long long)), "error report buffer");
+    unsigned long long h[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_correlate_error(P, rg, sv, out->data, out->stride, d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "error report");
+    double v[3];
+    memcpy(v, h, sizeof v);
+    if (max_abs) *max_abs = v[0];
+    if (max_rel) *max_rel = v[1];
+    if (max_scaled) *max_scaled = v[2];
+    if (cells) *cells = (int64_t)h[3];
+    g_err.clear();
+    return STRATA_OK;
+}
+
 int strata_get_tuning(strata_tuning *t)
 {
     if (!t) return fail(STRATA_EINVAL, "strata: nil tuning");

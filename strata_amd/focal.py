@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from .engine import EnginePanic, KernelDesc, Options, ProcessChunked, ProcessN
+from .raster import Float32Raster
 
 MaxRadius = 8
 
@@ -194,3 +195,25 @@ OPKERNELS = {
     "focal.Min": lambda o: newBox(o, "min"),
     "focal.Max": lambda o: newBox(o, "max"),
 }
+
+
+def CorrelateError(k: KernelDesc, src: Float32Raster, out: Float32Raster) -> dict:
+    """Error of a Correlate output against the FP64 correlation of src,
+    computed on the device over the interior cells whose window is finite:
+    max |out - ref|, max |out - ref| / |ref|, max |out - ref| / sum |w x|,
+    and the number of cells compared. For reporting the precision of the
+    FP32 path and of the split-BF16x3 MFMA path (newWeightsKernel(...,
+    mfma=True))."""
+    import ctypes
+    if not (src.on_device and out.on_device):
+        raise FocalPanic("focal: the error report reads device rasters")
+    kc, taps = k.to_c()
+    s, o = src.to_c(), out.to_c()
+    a, r, sc = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_int64()
+    rc = _lib.lib().strata_correlate_error(ctypes.byref(kc), ctypes.byref(s), ctypes.byref(o), ctypes.byref(a),
+                                           ctypes.byref(r), ctypes.byref(sc), ctypes.byref(n))
+    del taps
+    if rc < 0:
+        raise FocalPanic(_lib.last_error())
+    return {"max_abs": a.value, "max_rel": r.value, "max_scaled": sc.value, "cells": n.value}
