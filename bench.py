@@ -177,6 +177,13 @@ def main() -> int:
                 CellSize=30, Type=terrain.CurvatureMean)), [sink]),
             ("Gradient", terrain.gradientOp(terrain.GradientOptions(CellSize=30)), [sink, MemorySink(out2)]),
         ]
+        # terrain.Surface: Dx, Dy, Slope, Aspect and Hillshade from one read
+        # (five outputs, 24.75 B/cell with masks).
+        extra = [Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                               torch.zeros_like(valid), 0) for _ in range(3)]
+        ops.append(("Surface (Dx, Dy, Slope, Aspect, Hillshade)",
+                    terrain.newSurface([True] * 5, terrain.SurfaceOptions(CellSize=30, Azimuth=315, Altitude=45)),
+                    [sink, MemorySink(out2)] + [MemorySink(e) for e in extra]))
         suite["Slope"] = {"gcells_s": round(value, 2), "op_hbm_frac": round(op_bytes / (el / args.steps) / 1e9 /
                                                                               HBM_PEAK_GBS, 4),
                           "data_kernel_ms": round(kern_ms, 4), "validity_kernel_ms": round(vkern_ms, 4)}
@@ -189,7 +196,7 @@ def main() -> int:
                            "op_hbm_frac": round(b / (e2 / n_steps) / 1e9 / HBM_PEAK_GBS, 4),
                            "data_kernel_ms": round(t2.data_ms / max(1, t2.data_launches), 4)}
             log(f"bench: {name} {suite[name]}")
-        del out2
+        del out2, extra
 
     traffic = None
     if not args.no_traffic:

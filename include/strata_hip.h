@@ -74,6 +74,8 @@ enum strata_op {
     STRATA_OP_ASPECT = 3,     /* terrain.Aspect                                */
     STRATA_OP_HILLSHADE = 4,  /* terrain.Hillshade                             */
     STRATA_OP_CURVATURE = 5,  /* terrain.Curvature                             */
+    STRATA_OP_SURFACE = 6,    /* terrain.Surface: the STRATA_SURF_* products    */
+                              /* of one gradient, outputs in that order        */
     STRATA_OP_CORRELATE = 10, /* focal.Correlate / Convolve (weights pre-rotated) */
     STRATA_OP_SEPARABLE = 11, /* focal.CorrelateSeparable                      */
     STRATA_OP_MEAN = 12,      /* focal.Mean                                    */
@@ -88,6 +90,13 @@ enum strata_op {
 #define STRATA_FLAG_FIT 0x100  /* terrain op through Wood's quadratic fit,    */
                                /* radius = FitRadius (terrain/fit.go:187-226) */
 #define STRATA_CURV_SHIFT 4    /* curvature kind (0 profile, 1 plan, 2 mean)  */
+/* terrain.Surface products (terrain/surface.go:34-44), in output order */
+#define STRATA_SURF_DX 0x1000
+#define STRATA_SURF_DY 0x2000
+#define STRATA_SURF_SLOPE 0x4000
+#define STRATA_SURF_ASPECT 0x8000
+#define STRATA_SURF_HILLSHADE 0x10000
+#define STRATA_SURF_ALL 0x1f000
 #define STRATA_FLAG_MFMA 0x200 /* Correlate: split-BF16x3 MFMA path (meant for */
                                /* r >= 4). Approximate: each product within    */
                                /* about 3*2^-18 of |w*x|; windows holding a    */
@@ -101,6 +110,10 @@ enum strata_op {
  *   FIT (any terrain op): k0..k4 = Wood factors kp, kq, kr, kt, ks;
  *                         slope k5 scale; aspect k5 flat;
  *                         hillshade k5 c, k6 bx, k7 by
+ *   Surface (radius 1):   k0 kx, k1 ky, k2 slope scale, k3 aspect flat,
+ *                         k4 c, k5 bx, k6 by (ATAN, TRIG as for the ops)
+ *   Surface with FIT:     k0..k4 Wood factors, k5 slope scale, k6 aspect
+ *                         flat, k7 c, k8 bx, k9 by
  * taps (host pointer, copied during the call):
  *   CORRELATE: (2r+1)^2 weights, row-major; SEPARABLE: row[2r+1] then
  *   col[2r+1]; LOOKUP: xs[n] then ys[n] (ntaps = 2n);

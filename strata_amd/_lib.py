@@ -30,6 +30,7 @@ OP_SLOPE = 2
 OP_ASPECT = 3
 OP_HILLSHADE = 4
 OP_CURVATURE = 5
+OP_SURFACE = 6
 OP_CORRELATE = 10
 OP_SEPARABLE = 11
 OP_MEAN = 12
@@ -43,6 +44,11 @@ FLAG_TRIG = 0x2
 FLAG_FIT = 0x100
 CURV_SHIFT = 4
 FLAG_MFMA = 0x200
+SURF_DX = 0x1000
+SURF_DY = 0x2000
+SURF_SLOPE = 0x4000
+SURF_ASPECT = 0x8000
+SURF_HILLSHADE = 0x10000
 
 MAX_TAPS = 320
 

@@ -229,6 +229,23 @@ __device__ __forceinline__ void shade_pair(float gx0, float gy0, float gx1, floa
     o1 = v1;
 }
 
+// Horn's scaled gradient (internal/stencil/horn.go:172-180) of a lane's two
+// cells from rows A (y-1), B (y), C (y+1), grouped as the reference groups
+// it; the differences are shared between rows and columns.
+__device__ __forceinline__ void horn_grad(const HornArgs &a, const Row &A, const Row &B, const Row &C, float &gx0,
+                                          float &gy0, float &gx1, float &gy1)
+{
+    const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
+    const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
+    const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
+    const float dy0 = (gl + g1) + (g0 + g0);
+    const float dy1 = (g0 + gr) + (g1 + g1);
+    gx0 = dx0 * a.k[0];
+    gy0 = dy0 * a.k[1];
+    gx1 = dx1 * a.k[0];
+    gy1 = dy1 * a.k[1];
+}
+
 // The op over the two cells of a lane from rows A (y-1), B (y), C (y+1):
 // outputs (o00, o01) for the first output and (o10, o11) for the second.
 template <int OP>
@@ -240,15 +257,8 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
         o00 = zt_curvature(A.l, A.c0, A.c1, B.l, B.c0, B.c1, C.l, C.c0, C.c1, a.k, kind);
         o01 = zt_curvature(A.c0, A.c1, A.r, B.c0, B.c1, B.r, C.c0, C.c1, C.r, a.k, kind);
     } else {
-        // Horn (internal/stencil/horn.go:172-180), grouped as the reference
-        // groups it; the differences are shared between rows and columns.
-        const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
-        const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
-        const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
-        const float dy0 = (gl + g1) + (g0 + g0);
-        const float dy1 = (g0 + gr) + (g1 + g1);
-        const float gx0 = dx0 * a.k[0], gy0 = dy0 * a.k[1];
-        const float gx1 = dx1 * a.k[0], gy1 = dy1 * a.k[1];
+        float gx0, gy0, gx1, gy1;
+        horn_grad(a, A, B, C, gx0, gy0, gx1, gy1);
         if constexpr (OP == STRATA_OP_GRADIENT) {
             o00 = gx0;
             o01 = gx1;
@@ -275,10 +285,12 @@ struct NoPost {
 // maps every stored value (edge NaNs included) of the first output.
 // tid: the lane's index among the lanes sharing columns from xb (a block's
 // 256 or a wave's 64); each lane takes columns xb + 2*tid and xb + 2*tid + 1.
+// For STRATA_OP_SURFACE, dset holds one view per wanted product (a.flags'
+// STRATA_SURF_* bits, in output order) and d0, d1 are unused.
 template <int OP, bool PAIRS, class Post = NoPost>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
                                       const DstView &d1, int64_t xb, int64_t y0, int64_t y1, uint32_t tid,
-                                      const Post &post = Post())
+                                      const Post &post = Post(), const DstView *dset = nullptr)
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
     const int lane = (int)(tid & 63);
@@ -346,12 +358,39 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                 Row &D = R[(u + 3) & 3];
                 if (yy + 2 <= y1) load(D, yy + 2);
                 finish(C);
-                float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
-                horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
                 const bool ri = yy >= 1 && yy < rg.H - 1;
                 const float nan = edge_nan();
-                store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
-                if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
+                if constexpr (OP == STRATA_OP_SURFACE) {
+                    // terrain.Surface (terrain/surface.go:52-102): every wanted
+                    // product from one gradient, each with its op's arithmetic.
+                    float gx0, gy0, gx1, gy1;
+                    horn_grad(a, A, B, C, gx0, gy0, gx1, gy1);
+                    auto put = [&](int o, float v0, float v1) {
+                        store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
+                    };
+                    const int f = a.flags;
+                    int o = 0;
+                    if (f & STRATA_SURF_DX) put(o++, gx0, gx1);
+                    if (f & STRATA_SURF_DY) put(o++, gy0, gy1);
+                    if (f & STRATA_SURF_SLOPE) {
+                        float v0, v1;
+                        slope_pair(gx0, gy0, gx1, gy1, a.k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
+                        put(o++, v0, v1);
+                    }
+                    if (f & STRATA_SURF_ASPECT)
+                        put(o++, aspect_of(gx0, gy0, a.k[3], (f & STRATA_FLAG_TRIG) != 0),
+                            aspect_of(gx1, gy1, a.k[3], (f & STRATA_FLAG_TRIG) != 0));
+                    if (f & STRATA_SURF_HILLSHADE) {
+                        float v0, v1;
+                        shade_pair(gx0, gy0, gx1, gy1, a.k[4], a.k[5], a.k[6], v0, v1);
+                        put(o++, v0, v1);
+                    }
+                } else {
+                    float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
+                    horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
+                    store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
+                    if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
+                }
             }
         }
     }
@@ -381,6 +420,29 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
         sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
 }
 
+// The output views of one launch, by value in the kernel arguments.
+struct DstSet {
+    DstView v[kMaxOutputs];
+};
+
+// terrain.Surface at radius 1: horn_sweep's layout, every wanted product.
+__global__ __launch_bounds__(kSweepThreads) void horn_surface(HornArgs a, Region rg, SrcView s, DstSet d)
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
+    const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
+    const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
+    const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
+                       xb + kSweepCols <= rg.ox + rg.w;
+    if (pairs)
+        sweep<STRATA_OP_SURFACE, true>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
+    else
+        sweep<STRATA_OP_SURFACE, false>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
+}
+
 // ---------------------------------------------------------------------------
 // focal_tile: radius-r kernels through LDS
 // ---------------------------------------------------------------------------
@@ -404,7 +466,7 @@ enum FocalKind {
 // standalone launch, a device buffer for a batch).
 struct FocalArgs {
     int32_t op, flags;
-    float k[8];
+    float k[12];
 };
 struct ArgTaps {
     const Params &P;
@@ -424,10 +486,12 @@ constexpr int focal_lds_floats()
 }
 
 // One kTileX x kTileY output tile at (tx0, ty0) of the region.
+// For terrain.Surface through the fit (P.op STRATA_OP_SURFACE), dset holds
+// one view per wanted product, in output order.
 template <int KIND, int R, class TAPS>
 __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps, const Region &rg,
                                             const SrcView &s, const DstView &d0, const DstView &d1, int64_t tx0,
-                                            int64_t ty0, float *lds)
+                                            int64_t ty0, float *lds, const DstView *dset = nullptr)
 {
     constexpr int N = 2 * R + 1;
     constexpr int LW = kTileX + 2 * R;  // staged columns
@@ -556,6 +620,7 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
                 q = q * P.k[1];
                 switch (P.op) {
                 case STRATA_OP_GRADIENT:
+                case STRATA_OP_SURFACE:  // the products follow from p and q at the store
                     out0[o] = p;
                     out1[o] = q;
                     break;
@@ -597,6 +662,23 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         const int64_t y = ty0 + rbase + o;
         if (!col_in || y >= rg.oy + rg.h) continue;
         const bool interior = col_interior && y >= R && y < rg.H - R;
+        if constexpr (KIND == FK_FIT) {
+            if (P.op == STRATA_OP_SURFACE) {
+                // Each wanted product from the fit's gradient (p, q), with
+                // the k layout kp..ks, slope scale, aspect flat, c, bx, by.
+                const float p = out0[o], q = out1[o], nan = edge_nan();
+                auto put = [&](int i, float v) {
+                    gp(dset[i].p)[(y - rg.oy) * dset[i].stride + (x - rg.ox)] = interior ? v : nan;
+                };
+                int i = 0;
+                if (P.flags & STRATA_SURF_DX) put(i++, p);
+                if (P.flags & STRATA_SURF_DY) put(i++, q);
+                if (P.flags & STRATA_SURF_SLOPE) put(i++, slope_of(p, q, P.k[5], (P.flags & STRATA_FLAG_ATAN) != 0));
+                if (P.flags & STRATA_SURF_ASPECT) put(i++, aspect_of(p, q, P.k[6], (P.flags & STRATA_FLAG_TRIG) != 0));
+                if (P.flags & STRATA_SURF_HILLSHADE) put(i++, shade_of(p, q, P.k[7], P.k[8], P.k[9]));
+                continue;
+            }
+        }
         gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = interior ? out0[o] : edge_nan();
         if constexpr (KIND == FK_FIT) {
             if (P.op == STRATA_OP_GRADIENT)
@@ -606,18 +688,17 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
 }
 
 template <int KIND, int R>
-__global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Region rg, SrcView s, DstView d0,
-                                                           DstView d1)
+__global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Region rg, SrcView s, DstSet d)
 {
     __shared__ float lds[focal_lds_floats<KIND, R>()];
     FocalArgs a;
     a.op = P.op;
     a.flags = P.flags;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
+    for (int i = 0; i < 12; ++i) a.k[i] = P.k[i];
     const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kTileX;
     const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kTileY;
-    focal_block<KIND, R>(a, ArgTaps{P}, rg, s, d0, d1, tx0, ty0, lds);
+    focal_block<KIND, R>(a, ArgTaps{P}, rg, s, d.v[0], d.v[1], tx0, ty0, lds, d.v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,7 +1202,7 @@ __device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &
     a.op = op.op;
     a.flags = op.flags;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a.k[i] = op.k[i];
+    for (int i = 0; i < 12; ++i) a.k[i] = op.k[i];
     switch (op.kind) {
     case BK_MEAN: batch_focal_tiles<FK_MEAN, R>(a, t, lu, lds); break;
     case BK_MIN: batch_focal_tiles<FK_MIN, R>(a, t, lu, lds); break;
@@ -1272,7 +1353,7 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
                            hipStream_t st)
 {
     HornArgs a;
-    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
+    for (int i = 0; i < 12; ++i) a.k[i] = P.k[i];
     a.flags = P.flags;
     const DstView d1 = nout > 1 ? d[1] : d[0];
     // Pair (8-byte) accesses need every pair's first column at an even
@@ -1299,21 +1380,51 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
     return hipGetLastError();
 }
 
+static hipError_t run_surface(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
+                              hipStream_t st)
+{
+    HornArgs a;
+    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
+    a.flags = P.flags;
+    auto al8 = [](const void *p) { return ((uintptr_t)p & 7) == 0; };
+    bool aligned = al8(s.p) && (s.stride & 1) == 0 && ((rg.ox - s.sx) & 1) == 0;
+    for (int o = 0; o < nout; ++o) aligned = aligned && al8(d[o].p) && (d[o].stride & 1) == 0;
+    a.aligned = aligned ? 1 : 0;
+    a.rows = sweep_rows();
+    const int64_t gx = cdiv(rg.w, kSweepCols);
+    a.blocks_x = (uint32_t)gx;
+    const int64_t strips_per_launch = std::max<int64_t>(1, ((int64_t)1 << 30) / gx);
+    for (int64_t sb = 0; sb < cdiv(rg.h, a.rows); sb += strips_per_launch) {
+        Region sub = rg;
+        sub.oy = rg.oy + sb * a.rows;
+        sub.h = std::min<int64_t>(rg.h - sb * a.rows, strips_per_launch * a.rows);
+        DstSet e;
+        for (int o = 0; o < kMaxOutputs; ++o) {
+            e.v[o] = d[o < nout ? o : 0];
+            e.v[o].p += sb * a.rows * e.v[o].stride;
+        }
+        const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
+        hipLaunchKernelGGL(horn_surface, grid, dim3(kSweepThreads), 0, st, a, sub, s, e);
+    }
+    return hipGetLastError();
+}
+
 template <int KIND, int R>
 static hipError_t run_focal_r(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
                               hipStream_t st)
 {
-    const DstView d1 = nout > 1 ? d[1] : d[0];
     const int64_t gx = cdiv(rg.w, kTileX);
     for (int64_t yb = 0; yb < cdiv(rg.h, kTileY); yb += kMaxGridY) {
         Region sub = rg;
         sub.oy = rg.oy + yb * kTileY;
         sub.h = min(rg.h - yb * kTileY, kMaxGridY * kTileY);
-        DstView e0 = d[0], e1 = d1;
-        e0.p += yb * kTileY * e0.stride;
-        e1.p += yb * kTileY * e1.stride;
+        DstSet e;
+        for (int o = 0; o < kMaxOutputs; ++o) {
+            e.v[o] = d[o < nout ? o : 0];
+            e.v[o].p += yb * kTileY * e.v[o].stride;
+        }
         dim3 grid((unsigned)gx, (unsigned)cdiv(sub.h, kTileY));
-        hipLaunchKernelGGL((focal_tile<KIND, R>), grid, dim3(kTileThreads), 0, st, P, sub, s, e0, e1);
+        hipLaunchKernelGGL((focal_tile<KIND, R>), grid, dim3(kTileThreads), 0, st, P, sub, s, e);
     }
     return hipGetLastError();
 }
@@ -1419,9 +1530,10 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
 {
     if (rg.w <= 0 || rg.h <= 0) return hipSuccess;
     const bool fit = (P.flags & 0x100) != 0;
-    if (fit && P.op >= STRATA_OP_GRADIENT && P.op <= STRATA_OP_CURVATURE)
+    if (fit && P.op >= STRATA_OP_GRADIENT && P.op <= STRATA_OP_SURFACE)
         return run_focal<FK_FIT>(P, rg, s, d, nout, st);
     switch (P.op) {
+    case STRATA_OP_SURFACE: return run_surface(P, rg, s, d, nout, st);
     case STRATA_OP_GRADIENT: return run_horn<STRATA_OP_GRADIENT>(P, rg, s, d, nout, st);
     case STRATA_OP_SLOPE: return run_horn<STRATA_OP_SLOPE>(P, rg, s, d, nout, st);
     case STRATA_OP_ASPECT: return run_horn<STRATA_OP_ASPECT>(P, rg, s, d, nout, st);

@@ -14,6 +14,7 @@ namespace strata {
 
 constexpr int kMaxTaps = 320;   // 289 correlate weights (r = 8) fit comfortably
 constexpr int kMaxRadius = 8;   // focal.MaxRadius (focal/focal.go) and terrain.MaxRadius
+constexpr int kMaxOutputs = 5;  // terrain.Surface's products
 
 // The output region [ox, ox+w) x [oy, oy+h) of a W x H raster.
 struct Region {
@@ -64,7 +65,8 @@ Tuning get_tuning();
 bool set_tuning(const Tuning &t);
 
 // Data of every cell in the region: the op over interior cells, NaN over
-// the edge ring. nout is 1 or 2 (Gradient). Returns a HIP error code.
+// the edge ring. nout is 1, 2 (Gradient) or up to 5 (Surface). Returns a
+// HIP error code.
 hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
                        hipStream_t st);
 

@@ -55,13 +55,23 @@ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Kernel descriptors
 // ---------------------------------------------------------------------------
 
-int n_outputs(const strata_kernel *k) { return k->op == STRATA_OP_GRADIENT ? 2 : 1; }
+int n_outputs(const strata_kernel *k)
+{
+    if (k->op == STRATA_OP_GRADIENT) return 2;
+    if (k->op == STRATA_OP_SURFACE) return __builtin_popcount((unsigned)k->flags & STRATA_SURF_ALL);
+    return 1;
+}
 
 int check_kernel(const strata_kernel *k, int ndst, int nsrc)
 {
     if (!k) return fail(STRATA_EINVAL, "engine: nil kernel");
     const bool fit = (k->flags & STRATA_FLAG_FIT) != 0;
     switch (k->op) {
+    case STRATA_OP_SURFACE:
+        if ((k->flags & STRATA_SURF_ALL) == 0)
+            return fail(STRATA_EINVAL,
+                        "terrain: Surface has no outputs; set at least one of Dx, Dy, Slope, Aspect and Hillshade");
+        [[fallthrough]];
     case STRATA_OP_GRADIENT:
     case STRATA_OP_SLOPE:
     case STRATA_OP_ASPECT:
@@ -363,8 +373,8 @@ void tile_stats(strata_stats &s, int r, const TileGrid &g, int nin, int nout, co
 struct Operands {
     const strata_raster *src;
     int64_t sx, sy;
-    const strata_raster *dst[2];
-    int64_t dx[2], dy[2];
+    const strata_raster *dst[kMaxOutputs];
+    int64_t dx[kMaxOutputs], dy[kMaxOutputs];
     int nout;
     int64_t W, H;
 };
@@ -392,7 +402,7 @@ int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st
     Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
     if (rg.w <= 0 || rg.h <= 0) return STRATA_OK;
     const SrcView s = src_view(*op.src, op.sx, op.sy, op.W, op.H);
-    DstView d[2];
+    DstView d[kMaxOutputs];
     for (int o = 0; o < op.nout; ++o) {
         d[o].p = op.dst[o]->data + (rc.y0 - op.dy[o]) * op.dst[o]->stride + (rc.x0 - op.dx[o]);
         d[o].stride = op.dst[o]->stride;
@@ -625,9 +635,10 @@ int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, c
     // Host operands are mirrored on the device for the call.
     bool any_host = src[0].mem == STRATA_MEM_HOST;
     for (int o = 0; o < ndst; ++o) any_host = any_host || dst[o].mem == STRATA_MEM_HOST;
-    Staged ssrc, sdst[2];
+    Staged ssrc, sdst[kMaxOutputs];
     const strata_raster *dsrc = &src[0];
-    const strata_raster *ddst[2] = {&dst[0], ndst > 1 ? &dst[1] : &dst[0]};
+    const strata_raster *ddst[kMaxOutputs];
+    for (int o = 0; o < ndst; ++o) ddst[o] = &dst[o];
     if (src[0].mem == STRATA_MEM_HOST) {
         if ((rc = stage_in(src[0], ssrc, true, st))) return rc;
         dsrc = &ssrc.dev;
@@ -638,7 +649,8 @@ int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, c
             ddst[o] = &sdst[o].dev;
         }
 
-    Operands op{dsrc, 0, 0, {ddst[0], ddst[1]}, {0, 0}, {0, 0}, ndst, W, H};
+    Operands op{dsrc, 0, 0, {}, {}, {}, ndst, W, H};
+    for (int o = 0; o < ndst; ++o) op.dst[o] = ddst[o];
     strata_stats s{};
     const Plan p = new_plan(W, H, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
     const bool can_cancel = opts && opts->cancel;
@@ -780,7 +792,7 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
             // every tile reads its halo straight from HBM and writes in place,
             // so the queue is one launch over the raster (tiling never changes
             // a result) and the tiles' Stats follow in closed form.
-            strata_raster sv = *src[0].resident, dv[2]{};
+            strata_raster sv = *src[0].resident, dv[kMaxOutputs]{};
             if (sv.mem == STRATA_MEM_HOST) return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
             if (!src[0].masked) sv.valid = nullptr;
             for (int o = 0; o < ndst; ++o) {
@@ -789,7 +801,8 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
                     return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
                 if (!dst[o].masked) dv[o].valid = nullptr;
             }
-            Operands op{&sv, 0, 0, {&dv[0], &dv[ndst > 1 ? 1 : 0]}, {0, 0}, {0, 0}, ndst, W, H};
+            Operands op{&sv, 0, 0, {}, {}, {}, ndst, W, H};
+            for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
             if ((rc = run_rect(P, op, Rect{0, 0, W, H}, st))) return rc;
             for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, g, nsrc, ndst, tile_rect(g, t));
             if (!(opts && opts->async)) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
@@ -839,8 +852,9 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
                 sx = rx0;
                 sy = ry0;
             }
-            strata_raster dv[2]{};
-            Operands op{&sv, sx, sy, {&dv[0], &dv[ndst > 1 ? 1 : 0]}, {0, 0}, {0, 0}, ndst, W, H};
+            strata_raster dv[kMaxOutputs]{};
+            Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
+            for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
             bool buffered = !src[0].resident;
             for (int o = 0; o < ndst; ++o) {
                 if (dst[o].resident) {

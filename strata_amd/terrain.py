@@ -368,3 +368,94 @@ OPKERNELS = {
     "terrain.Hillshade": hillshadeOp,
     "terrain.Curvature": curvatureOp,
 }
+
+
+# -- Surface: several products from one read and one gradient ------------------------------
+
+@dataclass
+class SurfaceOptions:
+    """terrain.SurfaceOptions (terrain/surface.go:12-32)."""
+    CellSize: float = 0.0
+    CellSizeY: float = 0.0
+    ZFactor: float = 0.0
+    FitRadius: int = 0
+    Units: int = SlopeDegrees
+    ZeroForFlat: bool = False
+    Trigonometric: bool = False
+    Azimuth: float = 0.0
+    Altitude: float = 0.0
+
+
+@dataclass
+class SurfaceOutputs:
+    """terrain.SurfaceOutputs (terrain/surface.go:34-44): None is a product
+    not wanted; at least one must be set."""
+    Dx: Optional[Float32Raster] = None
+    Dy: Optional[Float32Raster] = None
+    Slope: Optional[Float32Raster] = None
+    Aspect: Optional[Float32Raster] = None
+    Hillshade: Optional[Float32Raster] = None
+
+
+@dataclass
+class SurfaceSinks:
+    """terrain.SurfaceSinks (terrain/surface.go:46-52) for SurfaceChunked."""
+    Dx: Optional[RasterSink] = None
+    Dy: Optional[RasterSink] = None
+    Slope: Optional[RasterSink] = None
+    Aspect: Optional[RasterSink] = None
+    Hillshade: Optional[RasterSink] = None
+
+
+_SURF_BITS = (_lib.SURF_DX, _lib.SURF_DY, _lib.SURF_SLOPE, _lib.SURF_ASPECT, _lib.SURF_HILLSHADE)
+
+
+def newSurface(want, o: SurfaceOptions) -> KernelDesc:
+    """The kernel of terrain.newSurface (terrain/surface.go:104-146): the
+    wanted products of one gradient, in SurfaceOutputs order. Every
+    product's options are checked, wanted or not, as the reference does."""
+    _check_fit(o.FitRadius)
+    scale, atan = slopeScale(o.Units)
+    flat = aspectFlat(o.ZeroForFlat)
+    c, bx, by = hillshadeLight(o.Azimuth, o.Altitude)
+    flags = 0
+    for w, bit in zip(want, _SURF_BITS):
+        if w:
+            flags |= bit
+    if not flags:
+        raise TerrainPanic("terrain: Surface has no outputs; set at least one of Dx, Dy, Slope, Aspect and Hillshade")
+    if atan:
+        flags |= _lib.FLAG_ATAN
+    if o.Trigonometric:
+        flags |= _lib.FLAG_TRIG
+    nout = sum(1 for w in want if w)
+    if o.FitRadius:
+        f = woodFit(o.FitRadius, o.CellSize, o.CellSizeY, o.ZFactor)
+        return KernelDesc(_lib.OP_SURFACE, o.FitRadius, flags | _lib.FLAG_FIT, list(f) + [scale, flat, c, bx, by],
+                          None, 1, nout, "terrain.Surface")
+    kx, ky = cellSizes(o.CellSize, o.CellSizeY, o.ZFactor)
+    return KernelDesc(_lib.OP_SURFACE, 1, flags, [kx, ky, scale, flat, c, bx, by], None, 1, nout,
+                      "terrain.Surface")
+
+
+def _surface_outs(out):
+    rasters = [out.Dx, out.Dy, out.Slope, out.Aspect, out.Hillshade]
+    return [r is not None for r in rasters], [r for r in rasters if r is not None]
+
+
+def Surface(out: SurfaceOutputs, dem, opts: SurfaceOptions) -> None:
+    """terrain.Surface (terrain/surface.go:53-68): any of dem's gradient,
+    slope, aspect and hillshade in one pass, each bit for bit what the
+    standalone op writes."""
+    want, outs = _surface_outs(out)
+    _run(newSurface(want, opts), dem, *outs)
+
+
+def SurfaceTiled(ctx, out: SurfaceOutputs, dem, opts: SurfaceOptions, eopts: Options):
+    want, outs = _surface_outs(out)
+    return _run_tiled(ctx, eopts, newSurface(want, opts), dem, *outs)
+
+
+def SurfaceChunked(ctx, out: SurfaceSinks, dem, opts: SurfaceOptions, eopts: Options):
+    want, outs = _surface_outs(out)
+    return _run_chunked(ctx, eopts, newSurface(want, opts), dem, *outs)
