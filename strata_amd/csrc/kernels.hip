@@ -334,7 +334,8 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
         STRATA_GLOBAL float *rp = gp(d.p) + (y - rg.oy) * d.stride;
         if constexpr (PAIRS) {
-            *reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so) = pair_t{v0, v1};
+            // Outputs are not read again by this launch: stream them.
+            __builtin_nontemporal_store(pair_t{v0, v1}, reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so));
         } else {
             if (in0) rp[so] = v0;
             if (in1) rp[so + 1] = v1;
@@ -1336,7 +1337,7 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
 
 // Launch-shape tuning (never changes a result).
-static Tuning g_tuning{64, {0, 0, 0, 0, 0, 0, 0}};
+static Tuning g_tuning{32, {0, 0, 0, 0, 0, 0, 0}};
 static int sweep_rows() { return g_tuning.sweep_rows; }
 
 Tuning get_tuning() { return g_tuning; }
