@@ -296,30 +296,38 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     const int lane = (int)(tid & 63);
     const uint32_t off = 2u * tid;  // the pair's first column, relative to xb
     const int64_t x = xb + off;
-    const bool halo_lane = lane == 0 || lane == 63;
-    // Offsets from a row's start of the cells this lane loads, clamped into
-    // the buffer: a clamped cell is read only for edge outputs, which become
-    // NaN. They are never negative, since the buffer starts at or after sx.
-    auto col = [&](int64_t c) { return (uint32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
+    const bool halo_lane = (uint32_t)(lane - 1) >= 62u;  // lanes 0 and 63
+    // Byte offsets from a row's start of the cells this lane loads, clamped
+    // into the buffer: a clamped cell is read only for edge outputs, which
+    // become NaN. They are never negative (the buffer starts at or after sx)
+    // and are 32-bit, so each access is a uniform row address plus a lane
+    // offset, with no per-lane 64-bit address arithmetic.
+    auto col = [&](int64_t c) { return 4u * (uint32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
     const uint32_t o0 = col(x), o1 = col(x + 1), oh = col(lane == 0 ? x - 1 : x + 2);
-    const uint32_t so = (uint32_t)(xb - rg.ox) + off;  // store offset from a region row's start
+    const uint32_t so = 4u * ((uint32_t)(xb - rg.ox) + off);  // store byte offset in a region row
     const bool ci0 = x >= 1 && x < rg.W - 1;
     const bool ci1 = x + 1 >= 1 && x + 1 < rg.W - 1;
     const bool in0 = x >= rg.ox && x < rg.ox + rg.w;
     const bool in1 = x + 1 >= rg.ox && x + 1 < rg.ox + rg.w;
+    typedef const STRATA_GLOBAL char *crow_t;
+    typedef STRATA_GLOBAL char *row_t;
 
-    auto load = [&](Row &row, int64_t y) {
-        const STRATA_GLOBAL float *rp = gp(s.p) + (min(max(y, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
+    // Source rows are clamped into the buffer in 32 bits relative to y0.
+    const int32_t rlo = (int32_t)min(max(s.ly0 - y0, (int64_t)-2), (int64_t)0);
+    const int32_t rhi = (int32_t)min(max(s.ly1 - 1 - y0, (int64_t)-2), (int64_t)4097);
+    auto load = [&](Row &row, int32_t u) {  // row y0 + u
+        const int32_t uc = min(max(u, rlo), rhi);
+        const crow_t rp = reinterpret_cast<crow_t>(gp(s.p) + (y0 + uc - s.sy) * s.stride);
         float c0, c1, h = 0.0f;
         if constexpr (PAIRS) {
             const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0);
             c0 = v.x;
             c1 = v.y;
         } else {
-            c0 = rp[o0];
-            c1 = rp[o1];
+            c0 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o0);
+            c1 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o1);
         }
-        if (halo_lane) h = rp[oh];
+        if (halo_lane) h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oh);
         row.c0 = c0;
         row.c1 = c1;
         row.l = h;  // the halo value until finish() forms l and r
@@ -332,34 +340,40 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
         row.hd1 = row.r - row.c0;
     };
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
-        STRATA_GLOBAL float *rp = gp(d.p) + (y - rg.oy) * d.stride;
+        const row_t rp = reinterpret_cast<row_t>(gp(d.p) + (y - rg.oy) * d.stride);
         if constexpr (PAIRS) {
             // Outputs are not read again by this launch: stream them.
             __builtin_nontemporal_store(pair_t{v0, v1}, reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so));
         } else {
-            if (in0) rp[so] = v0;
-            if (in1) rp[so + 1] = v1;
+            if (in0) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so) = v0;
+            if (in1) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so + 4u) = v1;
         }
     };
 
+    // Rows are counted from y0 in 32 bits (a strip is at most 4096 rows), so
+    // the loop tests and the interior test are scalar 32-bit compares.
+    const int32_t nrows = (int32_t)(y1 - y0);
+    const int32_t ilo = (int32_t)min(max(1 - y0, (int64_t)-1), (int64_t)nrows + 1);          // rows >= 1
+    const int32_t ihi = (int32_t)min(max(rg.H - 1 - y0, (int64_t)-1), (int64_t)nrows + 1);   // rows < H-1
     Row R[4];
-    load(R[0], y0 - 1);
-    load(R[1], y0);
-    load(R[2], y0 + 1);
+    load(R[0], -1);
+    load(R[1], 0);
+    load(R[2], 1);
     finish(R[0]);
     finish(R[1]);
-    for (int64_t y = y0; y < y1; y += 4) {
+    for (int32_t ub = 0; ub < nrows; ub += 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int64_t yy = y + u;
-            if (yy < y1) {
+            const int32_t uu = ub + u;
+            const int64_t yy = y0 + uu;
+            if (uu < nrows) {
                 Row &A = R[u & 3];
                 Row &B = R[(u + 1) & 3];
                 Row &C = R[(u + 2) & 3];
                 Row &D = R[(u + 3) & 3];
-                if (yy + 2 <= y1) load(D, yy + 2);
+                if (uu + 2 <= nrows) load(D, uu + 2);
                 finish(C);
-                const bool ri = yy >= 1 && yy < rg.H - 1;
+                const bool ri = uu >= ilo && uu < ihi;
                 const float nan = edge_nan();
                 if constexpr (OP == STRATA_OP_SURFACE) {
                     // terrain.Surface (terrain/surface.go:52-102): every wanted
