@@ -492,12 +492,21 @@ struct PtrTaps {
     __device__ float operator[](int i) const { return p[i]; }
 };
 
+// Correlate weights in LDS: rows of 2R+1 padded to a multiple of four, so a
+// weight row is a run of aligned 16-byte broadcast reads.
+template <int R>
+constexpr int weight_pitch()
+{
+    return (2 * R + 1 + 3) & ~3;
+}
+
 // LDS floats one focal_block<KIND, R> uses.
 template <int KIND, int R>
 constexpr int focal_lds_floats()
 {
     return (kTileY + 2 * R) * (kTileX + 2 * R) +
-           ((KIND == FK_CORRELATE) ? 0 : (KIND == FK_FIT ? 3 : 1) * kTileY * (kTileX + 2 * R));
+           ((KIND == FK_CORRELATE) ? (2 * R + 1) * weight_pitch<R>()
+                                   : (KIND == FK_FIT ? 3 : 1) * kTileY * (kTileX + 2 * R));
 }
 
 // One kTileX x kTileY output tile at (tx0, ty0) of the region.
@@ -521,6 +530,14 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         int row = i / LW, col = i - row * LW;
         tile[row][col] = load_cell(s, tx0 - R + col, ty0 - R + row);
     }
+    if constexpr (KIND == FK_CORRELATE && R > 2) {
+        constexpr int WP = weight_pitch<R>();
+        float *wts = lds + LH * LW;
+        for (int i = tid; i < N * WP; i += kTileThreads) {
+            const int j = i / WP, c = i - j * WP;
+            wts[i] = c < N ? taps[j * N + c] : 0.0f;
+        }
+    }
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
@@ -530,22 +547,48 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
 
     if constexpr (KIND == FK_CORRELATE) {
         // Per output: w[0]*v(0,0), then acc + w[t]*v(t) over the weights in
-        // row-major order (internal/focalrow/focalrow.go:217-230). Each LDS
-        // value is read once and applied to every output row that uses it;
-        // for each output the terms still arrive in row-major order.
+        // row-major order (internal/focalrow/focalrow.go:217-230). Each input
+        // row's 2R+1 values are read from LDS once into registers and
+        // applied to every output row that uses that input row; for each
+        // output the terms still arrive in row-major order. The sums start
+        // at -0: -0 + t == t exactly for every t (signed zeros and NaN
+        // included), so the first product starts the sum as in the
+        // reference, without a select.
+        constexpr int WP = weight_pitch<R>();
+        const float *wts = lds + LH * LW;
         float acc[kRowsPerWave];
 #pragma unroll
-        for (int o = 0; o < kRowsPerWave; ++o) acc[o] = 0.0f;
-        for (int row = 0; row < kRowsPerWave + 2 * R; ++row) {
+        for (int o = 0; o < kRowsPerWave; ++o) acc[o] = -0.0f;
+        if constexpr (R <= 2) {
+            // Few weights: the row loop unrolls and the weights are
+            // constants read from the kernel arguments.
 #pragma unroll
-            for (int c = 0; c < N; ++c) {
-                const float v = tile[rbase + row][lane + c];
+            for (int row = 0; row < kRowsPerWave + 2 * R; ++row) {
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const float v = tile[rbase + row][lane + c];
+#pragma unroll
+                    for (int o = 0; o < kRowsPerWave; ++o) {
+                        const int j = row - o;
+                        if (j >= 0 && j < N) acc[o] = acc[o] + taps[j * N + c] * v;
+                    }
+                }
+            }
+        } else {
+            // Not unrolled: unrolled, the weights of every row would be held
+            // in registers at once. Weight rows come from LDS.
+#pragma unroll 1
+            for (int row = 0; row < kRowsPerWave + 2 * R; ++row) {
+                float v[N];
+#pragma unroll
+                for (int c = 0; c < N; ++c) v[c] = tile[rbase + row][lane + c];
 #pragma unroll
                 for (int o = 0; o < kRowsPerWave; ++o) {
                     const int j = row - o;
-                    if (j >= 0 && j < N) {
-                        const float term = taps[j * N + c] * v;
-                        acc[o] = (j == 0 && c == 0) ? term : acc[o] + term;
+                    if (j >= 0 && j < N) {  // wave-uniform
+                        const float *wr = wts + j * WP;
+#pragma unroll
+                        for (int c = 0; c < N; ++c) acc[o] = acc[o] + wr[c] * v[c];
                     }
                 }
             }
@@ -773,7 +816,10 @@ __device__ __forceinline__ float correlate_cell(const float *taps, const SrcView
     constexpr int N = 2 * R + 1;
     float acc = 0.0f;
     nonfinite = false;
+    // The rare path: kept rolled so it does not raise the kernel's registers.
+#pragma unroll 1
     for (int j = 0; j < N; ++j)
+#pragma unroll 1
         for (int c = 0; c < N; ++c) {
             const float v = load_cell(s, x - R + c, y - R + j);
             nonfinite = nonfinite || !__builtin_isfinite(v);
@@ -799,6 +845,7 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
     __syncthreads();
     bool any = false;
+#pragma unroll 4
     for (int i = tid; i < LH * PITCH; i += 256) {
         const int row = i / PITCH, col = i - row * PITCH;
         float v = col < LW ? load_cell(s, tx0 - R + col, ty0 - R + row) : 0.0f;
@@ -816,7 +863,9 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     f32x4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
+    // Not unrolled: unrolled, every weight row's operands would be held in
+    // registers at once.
+#pragma unroll 1
     for (int i = 0; i < N; ++i) {
         bf16x8 bh, bl;
 #pragma unroll
