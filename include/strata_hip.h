@@ -241,6 +241,31 @@ int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width,
                       const strata_options *opts, int chunked, strata_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Fan-out: one input, several single-output kernels, one pass         */
+/* ------------------------------------------------------------------ */
+/*
+ * exec.ProcessN / exec.ProcessChunked with the kernel exec.NewPipeline(1,
+ * stages, outs) where stage i is parts[i] reading the one input and
+ * writing output i: the pipeline terrain.Features builds
+ * (terrain/features.go:103-117, called at features.go:76-98). Output i is
+ * bit for bit what parts[i] alone writes: its own edge ring (its radius)
+ * and its validity eroded by its own radius (internal/exec/kernel.go:
+ * 94-110, internal/exec/tile.go:155-184). The call's radius is the
+ * largest part radius; Stats follow the reference's band loop with the
+ * kernel run outside the narrowest ring (internal/exec/halo.go:21-57).
+ * Radius-1 Horn ops (Slope, Aspect, Hillshade, Curvature without FIT)
+ * share one read of the source. Each part must have one input and one
+ * output; at most 16 parts. Errors as strata_process.
+ */
+int strata_process_fanout(const strata_kernel *parts, int nparts, const strata_raster *dst,
+                          const strata_raster *src, const strata_options *opts, strata_stats *stats);
+int strata_process_fanout_chunked(const strata_kernel *parts, int nparts, const strata_sink *dst,
+                                  const strata_source *src, const strata_options *opts, strata_stats *stats);
+/* Closed-form Stats of a fan-out call, as strata_plan_stats. */
+int strata_plan_stats_fanout(const strata_kernel *parts, int nparts, int64_t width, int64_t height,
+                             const strata_options *opts, int chunked, strata_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Launch tuning (no counterpart in the reference): shapes only, never  */
 /* results                                                              */
 /* ------------------------------------------------------------------ */

@@ -211,6 +211,9 @@ EXPORTS = (
     "strata_process",
     "strata_process_chunked",
     "strata_plan_stats",
+    "strata_process_fanout",
+    "strata_process_fanout_chunked",
+    "strata_plan_stats_fanout",
     "strata_get_tuning",
     "strata_set_tuning",
     "strata_selfcheck",
@@ -262,6 +265,15 @@ def lib() -> ctypes.CDLL:
     L.strata_plan_stats.argtypes = [POINTER(Kernel), c_int, c_int, c_int64, c_int64, POINTER(Options), c_int,
                                     POINTER(Stats)]
     L.strata_plan_stats.restype = c_int
+    L.strata_process_fanout.argtypes = [POINTER(Kernel), c_int, POINTER(Raster), POINTER(Raster),
+                                        POINTER(Options), POINTER(Stats)]
+    L.strata_process_fanout.restype = c_int
+    L.strata_process_fanout_chunked.argtypes = [POINTER(Kernel), c_int, POINTER(Sink), POINTER(Source),
+                                                POINTER(Options), POINTER(Stats)]
+    L.strata_process_fanout_chunked.restype = c_int
+    L.strata_plan_stats_fanout.argtypes = [POINTER(Kernel), c_int, c_int64, c_int64, POINTER(Options), c_int,
+                                           POINTER(Stats)]
+    L.strata_plan_stats_fanout.restype = c_int
     L.strata_get_tuning.argtypes = [POINTER(Tuning)]
     L.strata_get_tuning.restype = c_int
     L.strata_set_tuning.argtypes = [POINTER(Tuning)]

@@ -287,10 +287,12 @@ struct NoPost {
 // 256 or a wave's 64); each lane takes columns xb + 2*tid and xb + 2*tid + 1.
 // For STRATA_OP_SURFACE, dset holds one view per wanted product (a.flags'
 // STRATA_SURF_* bits, in output order) and d0, d1 are unused.
+// For kOpHornFan, dset holds output o's view and fan its op.
 template <int OP, bool PAIRS, class Post = NoPost>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
                                       const DstView &d1, int64_t xb, int64_t y0, int64_t y1, uint32_t tid,
-                                      const Post &post = Post(), const DstView *dset = nullptr)
+                                      const Post &post = Post(), const DstView *dset = nullptr,
+                                      const FanArgs *fan = nullptr)
 {
     constexpr bool kTwo = OP == STRATA_OP_GRADIENT;
     const int lane = (int)(tid & 63);
@@ -400,6 +402,44 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                         shade_pair(gx0, gy0, gx1, gy1, a.k[4], a.k[5], a.k[6], v0, v1);
                         put(o++, v0, v1);
                     }
+                } else if constexpr (OP == kOpHornFan) {
+                    // terrain.Features (terrain/features.go:68-117): each output
+                    // is its own op with its own constants. The unscaled Horn
+                    // differences are the same for every op; each scales them
+                    // by its own kx, ky, as the op alone does.
+                    const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
+                    const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
+                    const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
+                    const float dy0 = (gl + g1) + (g0 + g0);
+                    const float dy1 = (g0 + gr) + (g1 + g1);
+#pragma unroll
+                    for (int o = 0; o < kMaxOutputs; ++o) {
+                        if (o < fan->n) {
+                            const float *k = fan->k[o];
+                            const int f = fan->flags[o];
+                            const float gx0 = dx0 * k[0], gy0 = dy0 * k[1];
+                            const float gx1 = dx1 * k[0], gy1 = dy1 * k[1];
+                            float v0, v1;
+                            switch (fan->op[o]) {
+                            case STRATA_OP_SLOPE:
+                                slope_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
+                                break;
+                            case STRATA_OP_ASPECT:
+                                v0 = aspect_of(gx0, gy0, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                                v1 = aspect_of(gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                                break;
+                            case STRATA_OP_HILLSHADE:
+                                shade_pair(gx0, gy0, gx1, gy1, k[2], k[3], k[4], v0, v1);
+                                break;
+                            default: {  // STRATA_OP_CURVATURE
+                                const int kind = (f >> STRATA_CURV_SHIFT) & 3;
+                                v0 = zt_curvature(A.l, A.c0, A.c1, B.l, B.c0, B.c1, C.l, C.c0, C.c1, k, kind);
+                                v1 = zt_curvature(A.c0, A.c1, A.r, B.c0, B.c1, B.r, C.c0, C.c1, C.r, k, kind);
+                            }
+                            }
+                            store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
+                        }
+                    }
                 } else {
                     float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
                     horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
@@ -456,6 +496,25 @@ __global__ __launch_bounds__(kSweepThreads) void horn_surface(HornArgs a, Region
         sweep<STRATA_OP_SURFACE, true>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
     else
         sweep<STRATA_OP_SURFACE, false>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
+}
+
+// A fan-out of radius-1 terrain ops: horn_sweep's layout, output o is
+// f.op[o] over the shared rows.
+__global__ __launch_bounds__(kSweepThreads) void horn_fan(HornArgs a, FanArgs f, Region rg, SrcView s, DstSet d)
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
+    const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
+    const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
+    const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
+                       xb + kSweepCols <= rg.ox + rg.w;
+    if (pairs)
+        sweep<kOpHornFan, true>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
+    else
+        sweep<kOpHornFan, false>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,7 +1476,7 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
                            hipStream_t st)
 {
     HornArgs a;
-    for (int i = 0; i < 12; ++i) a.k[i] = P.k[i];
+    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
     a.flags = P.flags;
     const DstView d1 = nout > 1 ? d[1] : d[0];
     // Pair (8-byte) accesses need every pair's first column at an even
@@ -1444,8 +1503,10 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
     return hipGetLastError();
 }
 
+// Surface (fan == false) or a fan-out of radius-1 ops (fan == true): one
+// sweep writing every output view of d.
 static hipError_t run_surface(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
-                              hipStream_t st)
+                              hipStream_t st, bool fan = false)
 {
     HornArgs a;
     for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
@@ -1468,7 +1529,10 @@ static hipError_t run_surface(const Params &P, const Region &rg, const SrcView &
             e.v[o].p += sb * a.rows * e.v[o].stride;
         }
         const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
-        hipLaunchKernelGGL(horn_surface, grid, dim3(kSweepThreads), 0, st, a, sub, s, e);
+        if (fan)
+            hipLaunchKernelGGL(horn_fan, grid, dim3(kSweepThreads), 0, st, a, P.fan, sub, s, e);
+        else
+            hipLaunchKernelGGL(horn_surface, grid, dim3(kSweepThreads), 0, st, a, sub, s, e);
     }
     return hipGetLastError();
 }
@@ -1598,6 +1662,7 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
         return run_focal<FK_FIT>(P, rg, s, d, nout, st);
     switch (P.op) {
     case STRATA_OP_SURFACE: return run_surface(P, rg, s, d, nout, st);
+    case kOpHornFan: return run_surface(P, rg, s, d, nout, st, true);
     case STRATA_OP_GRADIENT: return run_horn<STRATA_OP_GRADIENT>(P, rg, s, d, nout, st);
     case STRATA_OP_SLOPE: return run_horn<STRATA_OP_SLOPE>(P, rg, s, d, nout, st);
     case STRATA_OP_ASPECT: return run_horn<STRATA_OP_ASPECT>(P, rg, s, d, nout, st);

@@ -47,13 +47,28 @@ struct MaskView {
     int64_t lx0, lx1, ly0, ly1;
 };
 
+// Radius-1 terrain ops reading one source in one pass (a fan-out of
+// single-output ops, as terrain.Features builds, terrain/features.go:
+// 103-117): output o is op[o] (Slope, Aspect, Hillshade or Curvature)
+// with its own flags[o] and constants k[o][] (the strata_kernel layout).
+struct FanArgs {
+    int32_t n;
+    int32_t op[kMaxOutputs];
+    int32_t flags[kMaxOutputs];
+    float k[kMaxOutputs][8];
+};
+
+// Internal op id of a FanArgs launch (never crosses the C ABI).
+constexpr int32_t kOpHornFan = 100;
+
 // Resolved per-call constants: k[] as the C ABI's strata_kernel, taps
 // copied by value into the kernel arguments (read through the scalar
-// cache as uniform values).
+// cache as uniform values); fan for kOpHornFan.
 struct Params {
     int32_t op, radius, flags, ntaps;
     float k[16];
     float taps[kMaxTaps];
+    FanArgs fan;
 };
 
 // Launch-shape knobs (results never depend on them).

@@ -132,6 +132,111 @@ Params make_params(const strata_kernel *k)
 }
 
 // ---------------------------------------------------------------------------
+// Jobs: the device launches of one call
+// ---------------------------------------------------------------------------
+
+constexpr int kMaxDst = 16;  // outputs of one call (the parts of a fan-out)
+
+// One data launch writing outputs out[0..nout) of the call.
+struct Group {
+    Params P;
+    int nout = 0;
+    int out[kMaxOutputs] = {};
+};
+
+// A call's device work: its data launches, and each output's radius, which
+// is the width of its edge ring and the erosion of its validity. r is the
+// call's radius (how far its reads reach past an output cell) and rmin the
+// narrowest ring (internal/exec/tile.go:136-184).
+struct Job {
+    int nout = 0, r = 0, rmin = 0;
+    int radius[kMaxDst] = {};
+    std::vector<Group> groups;
+};
+
+Job make_job(const strata_kernel *k)
+{
+    Job J;
+    J.nout = n_outputs(k);
+    J.r = J.rmin = k->radius;
+    Group g;
+    g.P = make_params(k);
+    g.nout = J.nout;
+    for (int o = 0; o < J.nout; ++o) {
+        g.out[o] = o;
+        J.radius[o] = k->radius;
+    }
+    J.groups.push_back(g);
+    return J;
+}
+
+// Part i of a fan-out: a kernel of one input and one output.
+int check_parts(const strata_kernel *parts, int nparts)
+{
+    if (!parts || nparts < 1) return fail(STRATA_EINVAL, "engine: pipeline has no stages");
+    if (nparts > kMaxDst)
+        return fail(STRATA_EINVAL, "engine: %d outputs exceed the library's limit of %d", nparts, kMaxDst);
+    for (int i = 0; i < nparts; ++i) {
+        if (n_outputs(&parts[i]) != 1)
+            return fail(STRATA_EINVAL, "engine: stage %d has %d outputs; a fan-out stage has one", i,
+                        n_outputs(&parts[i]));
+        int rc = check_kernel(&parts[i], 1, 1);
+        if (rc) return rc;
+    }
+    return STRATA_OK;
+}
+
+// exec.NewPipeline(1, stages, outs) with stage i reading input 0 and
+// writing output i (terrain/features.go:103-117): the call's radius is the
+// largest stage radius, and each output keeps its own (its reach,
+// internal/exec/kernel.go:94-110). Radius-1 Horn ops share one sweep per
+// kMaxOutputs of them; every other stage is a launch of its own.
+Job make_fan_job(const strata_kernel *parts, int nparts)
+{
+    Job J;
+    J.nout = nparts;
+    J.r = 0;
+    J.rmin = kMaxRadius;
+    Group fan;
+    auto flush = [&]() {
+        if (fan.nout == 0) return;
+        fan.P.fan.n = fan.nout;
+        J.groups.push_back(fan);
+        fan = Group();
+    };
+    for (int i = 0; i < nparts; ++i) {
+        const strata_kernel &k = parts[i];
+        J.radius[i] = k.radius;
+        J.r = std::max(J.r, k.radius);
+        J.rmin = std::min(J.rmin, k.radius);
+        const bool horn = (k.op == STRATA_OP_SLOPE || k.op == STRATA_OP_ASPECT || k.op == STRATA_OP_HILLSHADE ||
+                           k.op == STRATA_OP_CURVATURE) &&
+                          !(k.flags & STRATA_FLAG_FIT);
+        if (!horn) {
+            Group g;
+            g.P = make_params(&k);
+            g.nout = 1;
+            g.out[0] = i;
+            J.groups.push_back(g);
+            continue;
+        }
+        if (fan.nout == 0) {
+            memset(&fan.P, 0, sizeof fan.P);
+            fan.P.op = kOpHornFan;
+            fan.P.radius = 1;
+        }
+        const int j = fan.nout++;
+        fan.out[j] = i;
+        fan.P.fan.op[j] = k.op;
+        fan.P.fan.flags[j] = k.flags;
+        memcpy(fan.P.fan.k[j], k.k, sizeof fan.P.fan.k[j]);
+        if (fan.nout == kMaxOutputs) flush();
+    }
+    flush();
+    return J;
+}
+
+// ---------------------------------------------------------------------------
 // Rasters
 // ---------------------------------------------------------------------------
 
@@ -287,12 +392,14 @@ Rect plan_band(const Plan &p, int64_t i)
 }
 
 // One band's counters (internal/exec/halo.go:22-39, 97-105), for the
-// band [x0, x1) x [y0, y1) of a W x H raster.
-void band_stats(strata_stats &s, int r, int64_t W, int64_t H, int nin, int nout, const Rect &b)
+// band [x0, x1) x [y0, y1) of a W x H raster: the kernel runs over the
+// cells outside the narrowest edge ring (rmin) and reads its window grown
+// by the radius r.
+void band_stats(strata_stats &s, int r, int rmin, int64_t W, int64_t H, int nin, int nout, const Rect &b)
 {
     const int64_t cells = (b.x1 - b.x0) * (b.y1 - b.y0);
-    const int64_t ix0 = std::max<int64_t>(b.x0, r), ix1 = std::min<int64_t>(b.x1, W - r);
-    const int64_t iy0 = std::max<int64_t>(b.y0, r), iy1 = std::min<int64_t>(b.y1, H - r);
+    const int64_t ix0 = std::max<int64_t>(b.x0, rmin), ix1 = std::min<int64_t>(b.x1, W - rmin);
+    const int64_t iy0 = std::max<int64_t>(b.y0, rmin), iy1 = std::min<int64_t>(b.y1, H - rmin);
     const bool interior = ix0 < ix1 && iy0 < iy1;
     const int64_t inner = interior ? (ix1 - ix0) * (iy1 - iy0) : 0;
     s.cells += cells;
@@ -346,7 +453,7 @@ Rect tile_rect(const TileGrid &g, int64_t i)
 
 // One chunked tile's counters (chunked.go:335-374): the source window,
 // the sink tile, and the tile's own plan of bands.
-void tile_stats(strata_stats &s, int r, const TileGrid &g, int nin, int nout, const Rect &t)
+void tile_stats(strata_stats &s, int r, int rmin, const TileGrid &g, int nin, int nout, const Rect &t)
 {
     const int64_t rx0 = std::max<int64_t>(0, t.x0 - r), ry0 = std::max<int64_t>(0, t.y0 - r);
     const int64_t rx1 = std::min<int64_t>(g.W, t.x1 + r), ry1 = std::min<int64_t>(g.H, t.y1 + r);
@@ -359,7 +466,7 @@ void tile_stats(strata_stats &s, int r, const TileGrid &g, int nin, int nout, co
         band.x1 += t.x0;
         band.y0 += t.y0;
         band.y1 += t.y0;
-        band_stats(s, r, g.W, g.H, nin, nout, band);
+        band_stats(s, r, rmin, g.W, g.H, nin, nout, band);
     }
     s.sink_written += (t.x1 - t.x0) * (t.y1 - t.y0) * kBytesPerCell * nout;
 }
@@ -373,8 +480,8 @@ void tile_stats(strata_stats &s, int r, const TileGrid &g, int nin, int nout, co
 struct Operands {
     const strata_raster *src;
     int64_t sx, sy;
-    const strata_raster *dst[kMaxOutputs];
-    int64_t dx[kMaxOutputs], dy[kMaxOutputs];
+    const strata_raster *dst[kMaxDst];
+    int64_t dx[kMaxDst], dy[kMaxDst];
     int nout;
     int64_t W, H;
 };
@@ -397,22 +504,25 @@ hipEvent_t timing_event(hipStream_t st)
     return e;
 }
 
-int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st)
+int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
 {
     Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
     if (rg.w <= 0 || rg.h <= 0) return STRATA_OK;
     const SrcView s = src_view(*op.src, op.sx, op.sy, op.W, op.H);
-    DstView d[kMaxOutputs];
-    for (int o = 0; o < op.nout; ++o) {
-        d[o].p = op.dst[o]->data + (rc.y0 - op.dy[o]) * op.dst[o]->stride + (rc.x0 - op.dx[o]);
-        d[o].stride = op.dst[o]->stride;
-    }
     const bool timed = g_timing.on;
-    hipEvent_t e0 = timed ? timing_event(st) : nullptr;
-    HIP_TRY(launch_data(P, rg, s, d, op.nout, st), "launching the data kernel");
-    if (timed) {
-        std::lock_guard<std::mutex> lk(g_timing_mu);
-        g_timing.data.emplace_back(e0, timing_event(st));
+    for (const Group &g : J.groups) {
+        DstView d[kMaxOutputs];
+        for (int i = 0; i < g.nout; ++i) {
+            const int o = g.out[i];
+            d[i].p = op.dst[o]->data + (rc.y0 - op.dy[o]) * op.dst[o]->stride + (rc.x0 - op.dx[o]);
+            d[i].stride = op.dst[o]->stride;
+        }
+        hipEvent_t e0 = timed ? timing_event(st) : nullptr;
+        HIP_TRY(launch_data(g.P, rg, s, d, g.nout, st), "launching the data kernel");
+        if (timed) {
+            std::lock_guard<std::mutex> lk(g_timing_mu);
+            g_timing.data.emplace_back(e0, timing_event(st));
+        }
     }
     for (int o = 0; o < op.nout; ++o) {
         if (!op.dst[o]->valid) continue;
@@ -420,7 +530,7 @@ int run_rect(const Params &P, const Operands &op, const Rect &rc, hipStream_t st
         const bool has_src = op.src->valid != nullptr;
         const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy, op.W, op.H) : dm;
         hipEvent_t v0 = timed ? timing_event(st) : nullptr;
-        HIP_TRY(launch_validity(P.radius, rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");
+        HIP_TRY(launch_validity(J.radius[o], rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");
         if (timed) {
             std::lock_guard<std::mutex> lk(g_timing_mu);
             g_timing.validity.emplace_back(v0, timing_event(st));
@@ -435,11 +545,10 @@ bool cancelled(const strata_options *o) { return o && o->cancel && *o->cancel !=
 // plan, within one tile, at least this many cells.
 constexpr int64_t kUnitCells = 1 << 22;
 
-int check_operands(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
+int check_operands(const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
                    const strata_options *opts)
 {
-    int rc = check_kernel(k, ndst, nsrc);
-    if (rc) return rc;
+    int rc = STRATA_OK;
     if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
     if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
         return fail(STRATA_EINVAL, "engine: negative Options {TileWidth:%lld TileHeight:%lld Workers:%lld}",
@@ -602,42 +711,45 @@ int strata_timing_end(double *data_ms, int64_t *data_launches, double *validity_
     return STRATA_OK;
 }
 
-int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
-                      const strata_options *opts, int chunked, strata_stats *stats)
+} // extern "C"
+
+namespace {
+
+int plan_stats(const Job &J, int nsrc, int64_t width, int64_t height, const strata_options *opts, int chunked,
+               strata_stats *stats)
 {
-    int rc = check_kernel(k, ndst, nsrc);
-    if (rc) return rc;
+    const int ndst = J.nout;
     if (width <= 0 || height <= 0) return fail(STRATA_EINVAL, "engine: sizes must be positive");
     strata_stats s{};
     if (chunked) {
         const TileGrid g = tile_grid(width, height, opts);
-        for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, k->radius, g, nsrc, ndst, tile_rect(g, t));
+        for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, J.r, J.rmin, g, nsrc, ndst, tile_rect(g, t));
     } else {
         const Plan p = new_plan(width, height, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
-        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, k->radius, width, height, nsrc, ndst, plan_band(p, b));
+        for (int64_t b = 0; b < p.bands; ++b)
+            band_stats(s, J.r, J.rmin, width, height, nsrc, ndst, plan_band(p, b));
     }
     add_stats(stats, s, nsrc, ndst);
     g_err.clear();
     return STRATA_OK;
 }
 
-int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
-                   const strata_options *opts, strata_stats *stats)
+int process(const Job &J, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
+            const strata_options *opts, strata_stats *stats)
 {
-    int rc = check_operands(k, dst, ndst, src, nsrc, opts);
+    int rc = check_operands(dst, ndst, src, nsrc, opts);
     if (rc) return rc;
     if ((rc = ensure_device())) return rc;
     hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
     const int64_t W = dst[0].width, H = dst[0].height;
-    const int r = k->radius;
-    const Params P = make_params(k);
+    const int r = J.r, rmin = J.rmin;
 
     // Host operands are mirrored on the device for the call.
     bool any_host = src[0].mem == STRATA_MEM_HOST;
     for (int o = 0; o < ndst; ++o) any_host = any_host || dst[o].mem == STRATA_MEM_HOST;
-    Staged ssrc, sdst[kMaxOutputs];
+    Staged ssrc, sdst[kMaxDst];
     const strata_raster *dsrc = &src[0];
-    const strata_raster *ddst[kMaxOutputs];
+    const strata_raster *ddst[kMaxDst];
     for (int o = 0; o < ndst; ++o) ddst[o] = &dst[o];
     if (src[0].mem == STRATA_MEM_HOST) {
         if ((rc = stage_in(src[0], ssrc, true, st))) return rc;
@@ -657,8 +769,8 @@ int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, c
     int status = STRATA_OK;
     if (!can_cancel) {
         // The plan never changes a result, so the whole raster is one launch.
-        if ((rc = run_rect(P, op, Rect{0, 0, W, H}, st))) return rc;
-        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, r, W, H, nsrc, ndst, plan_band(p, b));
+        if ((rc = run_rect(J, op, Rect{0, 0, W, H}, st))) return rc;
+        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, r, rmin, W, H, nsrc, ndst, plan_band(p, b));
     } else {
         // Units of whole bands, in plan order, within one tile; the check
         // comes before each claim and a claimed unit always completes.
@@ -670,17 +782,17 @@ int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, c
             }
             Rect u = plan_band(p, b);
             strata_stats us{};
-            band_stats(us, r, W, H, nsrc, ndst, u);
+            band_stats(us, r, rmin, W, H, nsrc, ndst, u);
             int64_t e = b + 1;
             while (e < p.bands) {
                 const Rect nb = plan_band(p, e);
                 if (nb.x0 != u.x0 || nb.x1 != u.x1 || nb.y0 != u.y1) break;
                 if ((u.x1 - u.x0) * (u.y1 - u.y0) >= kUnitCells) break;
-                band_stats(us, r, W, H, nsrc, ndst, nb);
+                band_stats(us, r, rmin, W, H, nsrc, ndst, nb);
                 u.y1 = nb.y1;
                 ++e;
             }
-            if ((rc = run_rect(P, op, u, st))) return rc;
+            if ((rc = run_rect(J, op, u, st))) return rc;
             HIP_TRY(hipStreamSynchronize(st), "waiting for a unit");
             s.cells += us.cells;
             s.bands += us.bands;
@@ -697,11 +809,10 @@ int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, c
     return status;
 }
 
-int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int ndst, const strata_source *src,
-                           int nsrc, const strata_options *opts, strata_stats *stats)
+int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata_source *src, int nsrc,
+                    const strata_options *opts, strata_stats *stats)
 {
-    int rc = check_kernel(k, ndst, nsrc);
-    if (rc) return rc;
+    int rc = STRATA_OK;
     if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
     if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
         return fail(STRATA_EINVAL, "engine: negative Options");
@@ -733,8 +844,7 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
     }
     if ((rc = ensure_device())) return rc;
     hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
-    const int r = k->radius;
-    const Params P = make_params(k);
+    const int r = J.r, rmin = J.rmin;
     const TileGrid g = tile_grid(W, H, opts);
     const bool opts_cancel = opts && opts->cancel;
 
@@ -792,7 +902,7 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
             // every tile reads its halo straight from HBM and writes in place,
             // so the queue is one launch over the raster (tiling never changes
             // a result) and the tiles' Stats follow in closed form.
-            strata_raster sv = *src[0].resident, dv[kMaxOutputs]{};
+            strata_raster sv = *src[0].resident, dv[kMaxDst]{};
             if (sv.mem == STRATA_MEM_HOST) return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
             if (!src[0].masked) sv.valid = nullptr;
             for (int o = 0; o < ndst; ++o) {
@@ -803,8 +913,8 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
             }
             Operands op{&sv, 0, 0, {}, {}, {}, ndst, W, H};
             for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
-            if ((rc = run_rect(P, op, Rect{0, 0, W, H}, st))) return rc;
-            for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, g, nsrc, ndst, tile_rect(g, t));
+            if ((rc = run_rect(J, op, Rect{0, 0, W, H}, st))) return rc;
+            for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, rmin, g, nsrc, ndst, tile_rect(g, t));
             if (!(opts && opts->async)) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
             add_stats(stats, s, nsrc, ndst);
             g_err.clear();
@@ -852,7 +962,7 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
                 sx = rx0;
                 sy = ry0;
             }
-            strata_raster dv[kMaxOutputs]{};
+            strata_raster dv[kMaxDst]{};
             Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
             for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
             bool buffered = !src[0].resident;
@@ -875,7 +985,7 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
                     buffered = true;
                 }
             }
-            if ((rc = run_rect(P, op, tr, st))) return rc;
+            if ((rc = run_rect(J, op, tr, st))) return rc;
             // Hand buffered outputs to their sinks.
             bool waited = false;
             for (int o = 0; o < ndst; ++o) {
@@ -904,13 +1014,65 @@ int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int n
             // A callback source's pinned buffer is reused by the next tile,
             // and a cancellable call observes its flag per finished tile.
             if (!waited && (buffered || opts_cancel)) HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
-            tile_stats(s, r, g, nsrc, ndst, tr);
+            tile_stats(s, r, rmin, g, nsrc, ndst, tr);
         }
         HIP_TRY(hipStreamSynchronize(st), "synchronizing");
         add_stats(stats, s, nsrc, ndst);
     }
     if (status == STRATA_OK) g_err.clear();
     return status;
+}
+
+} // namespace
+
+extern "C" {
+
+int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
+                      const strata_options *opts, int chunked, strata_stats *stats)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    return plan_stats(make_job(k), nsrc, width, height, opts, chunked, stats);
+}
+
+int strata_process(const strata_kernel *k, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
+                   const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    return process(make_job(k), dst, ndst, src, nsrc, opts, stats);
+}
+
+int strata_process_chunked(const strata_kernel *k, const strata_sink *dst, int ndst, const strata_source *src,
+                           int nsrc, const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_kernel(k, ndst, nsrc);
+    if (rc) return rc;
+    return process_chunked(make_job(k), dst, ndst, src, nsrc, opts, stats);
+}
+
+int strata_plan_stats_fanout(const strata_kernel *parts, int nparts, int64_t width, int64_t height,
+                             const strata_options *opts, int chunked, strata_stats *stats)
+{
+    int rc = check_parts(parts, nparts);
+    if (rc) return rc;
+    return plan_stats(make_fan_job(parts, nparts), 1, width, height, opts, chunked, stats);
+}
+
+int strata_process_fanout(const strata_kernel *parts, int nparts, const strata_raster *dst,
+                          const strata_raster *src, const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_parts(parts, nparts);
+    if (rc) return rc;
+    return process(make_fan_job(parts, nparts), dst, nparts, src, 1, opts, stats);
+}
+
+int strata_process_fanout_chunked(const strata_kernel *parts, int nparts, const strata_sink *dst,
+                                  const strata_source *src, const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_parts(parts, nparts);
+    if (rc) return rc;
+    return process_chunked(make_fan_job(parts, nparts), dst, nparts, src, 1, opts, stats);
 }
 
 } // extern "C"

@@ -158,6 +158,38 @@ class KernelDesc:
         return kc, taps  # keep taps alive for the call
 
 
+class FanOut:
+    """exec.NewPipeline(1, stages, outs) where stage i is parts[i] reading
+    the pipeline's one input and writing output i: the pipeline
+    terrain.newFeatures builds (terrain/features.go:103-117). The radius
+    is the largest part's; each output keeps its own ring and validity
+    (internal/exec/kernel.go:94-110)."""
+
+    def __init__(self, parts: Sequence[KernelDesc], name: str = "engine.Pipeline"):
+        if not parts:
+            raise EnginePanic("engine: pipeline has no stages")
+        for i, p in enumerate(parts):
+            if p is None:
+                raise EnginePanic(f"engine: stage {i} has a nil kernel")
+            if p.nin != 1 or p.nout != 1:
+                raise EnginePanic(f"engine: stage {i} takes {p.nin} inputs and {p.nout} outputs; a fan-out "
+                                  "stage takes one of each")
+        self.parts = list(parts)
+        self.radius = max(p.radius for p in self.parts)
+        self.nin = 1
+        self.nout = len(self.parts)
+        self.name = name
+
+    def to_c(self):
+        arr = (_lib.Kernel * len(self.parts))()
+        keep = []
+        for i, p in enumerate(self.parts):
+            kc, taps = p.to_c()
+            arr[i] = kc
+            keep.append(taps)
+        return arr, keep
+
+
 # -- overlap checks (internal/overlap/overlap.go) ------------------------------
 
 def _span(r: Float32Raster) -> int:
@@ -320,7 +352,10 @@ def ProcessN(ctx: Optional[Context], dst: List[Float32Raster], src: List[Float32
     cs = (_lib.Raster * len(src))(*[s.to_c() for s in src])
     st = _lib.Stats()
     co = _c_options(opts, ctx if (ctx is not None and not async_) else None, _stream_handle(dst + src), async_)
-    rc = L.strata_process(ctypes.byref(kc), cd, len(dst), cs, len(src), ctypes.byref(co), ctypes.byref(st))
+    if isinstance(k, FanOut):
+        rc = L.strata_process_fanout(kc, len(k.parts), cd, cs, ctypes.byref(co), ctypes.byref(st))
+    else:
+        rc = L.strata_process(ctypes.byref(kc), cd, len(dst), cs, len(src), ctypes.byref(co), ctypes.byref(st))
     del taps
     if rc < 0:
         _raise_status(rc)
@@ -338,8 +373,12 @@ def PlanStats(k: KernelDesc, width: int, height: int, opts: Optional[Options] = 
     kc, taps = k.to_c()
     co = _c_options(opts, None, 0)
     st = _lib.Stats()
-    rc = _lib.lib().strata_plan_stats(ctypes.byref(kc), k.nout, k.nin, width, height, ctypes.byref(co),
-                                      1 if chunked else 0, ctypes.byref(st))
+    if isinstance(k, FanOut):
+        rc = _lib.lib().strata_plan_stats_fanout(kc, len(k.parts), width, height, ctypes.byref(co),
+                                                 1 if chunked else 0, ctypes.byref(st))
+    else:
+        rc = _lib.lib().strata_plan_stats(ctypes.byref(kc), k.nout, k.nin, width, height, ctypes.byref(co),
+                                          1 if chunked else 0, ctypes.byref(st))
     if rc < 0:
         _raise_status(rc)
     out = Stats()
@@ -585,8 +624,12 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
     rasters = [x.r for x in list(src) + list(dst) if isinstance(x, (MemorySource, MemorySink))]
     co = _c_options(opts, ctx if not async_ else None, _stream_handle(rasters), async_)
     st = _lib.Stats()
-    rc = _lib.lib().strata_process_chunked(ctypes.byref(kc), cdst, len(dst), csrc, len(src), ctypes.byref(co),
-                                           ctypes.byref(st))
+    if isinstance(k, FanOut):
+        rc = _lib.lib().strata_process_fanout_chunked(kc, len(k.parts), cdst, csrc, ctypes.byref(co),
+                                                      ctypes.byref(st))
+    else:
+        rc = _lib.lib().strata_process_chunked(ctypes.byref(kc), cdst, len(dst), csrc, len(src),
+                                               ctypes.byref(co), ctypes.byref(st))
     del keep, taps
     if opts.Stats is not None:
         opts.Stats._add_c(st)

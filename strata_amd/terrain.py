@@ -20,12 +20,12 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass
 from enum import IntEnum
-from typing import Optional
+from typing import List, Optional
 
 import numpy as np
 
 from . import _lib
-from .engine import (Background, Context, EnginePanic, KernelDesc, Options, ProcessChunked, ProcessN,
+from .engine import (Background, Context, EnginePanic, FanOut, KernelDesc, Options, ProcessChunked, ProcessN,
                      RasterSink, RasterSource)
 from .raster import Float32Raster
 
@@ -459,3 +459,68 @@ def SurfaceTiled(ctx, out: SurfaceOutputs, dem, opts: SurfaceOptions, eopts: Opt
 def SurfaceChunked(ctx, out: SurfaceSinks, dem, opts: SurfaceOptions, eopts: Options):
     want, outs = _surface_outs(out)
     return _run_chunked(ctx, eopts, newSurface(want, opts), dem, *outs)
+
+
+# -- Features: a feature stack, each output its own op, from one read ------------------------
+
+@dataclass
+class Feature:
+    """terrain.Feature (terrain/features.go:29-34): an operation — one of
+    SlopeOptions, AspectOptions, HillshadeOptions and CurvatureOptions,
+    each meaning what it means to its own function — and the raster its
+    result goes to."""
+    Op: object
+    Dst: Float32Raster
+
+
+@dataclass
+class FeatureSink:
+    """terrain.FeatureSink (terrain/features.go:36-40) for FeaturesChunked."""
+    Op: object
+    Dst: RasterSink
+
+
+def _feature_kernel(i: int, op) -> KernelDesc:
+    # FeatureOp.featureKernel (terrain/features.go:21-27). HeatLoad,
+    # Orientation and Ruggedness are operations this library does not run.
+    if op is None:
+        raise TerrainPanic(f"terrain: Features output {i} has no Op")
+    if isinstance(op, SlopeOptions):
+        return slopeOp(op)
+    if isinstance(op, AspectOptions):
+        return aspectOp(op)
+    if isinstance(op, HillshadeOptions):
+        return hillshadeOp(op)
+    if isinstance(op, CurvatureOptions):
+        return curvatureOp(op)
+    raise TerrainPanic(f"terrain: Features output {i} has Op {type(op).__name__}, which is not a FeatureOp this "
+                       "library runs (Slope, Aspect, Hillshade and Curvature options are)")
+
+
+def newFeatures(ops) -> FanOut:
+    """terrain.newFeatures (terrain/features.go:103-117): the pipeline that
+    fans the DEM out to one stage per feature; output i is stage i."""
+    if len(ops) == 0:
+        raise TerrainPanic("terrain: Features has no outputs")
+    return FanOut([_feature_kernel(i, op) for i, op in enumerate(ops)], "terrain.Features")
+
+
+def Features(out: List[Feature], dem) -> None:
+    """terrain.Features (terrain/features.go:42-71): several terrain
+    features of one DEM in one pass over it, each bit for bit what its own
+    function writes with the same options: its own edge ring and its
+    validity eroded by its own window."""
+    FeaturesTiled(None, out, dem, Options(Workers=1))
+
+
+def FeaturesTiled(ctx, out: List[Feature], dem, eopts: Options):
+    """terrain.FeaturesTiled (terrain/features.go:73-86)."""
+    k = newFeatures([f.Op for f in out])
+    return ProcessN(ctx, [f.Dst for f in out], [dem], k, eopts)
+
+
+def FeaturesChunked(ctx, out: List[FeatureSink], dem: RasterSource, eopts: Options):
+    """terrain.FeaturesChunked (terrain/features.go:88-101): tiles read
+    grown by the largest radius asked for."""
+    k = newFeatures([f.Op for f in out])
+    return ProcessChunked(ctx, [f.Dst for f in out], [dem], k, eopts)

@@ -49,10 +49,13 @@ def plan_bands(w, h, tw, th):
     return out
 
 
-def band_stats(s, r, W, H, nin, nout, b):
+def band_stats(s, r, W, H, nin, nout, b, rmin=None):
+    # job.band (internal/exec/halo.go:21-57): the kernel runs outside the
+    # narrowest edge ring, rmin, and reads its window grown by r.
+    lo = r if rmin is None else rmin
     x0, y0, x1, y1 = b
     cells = (x1 - x0) * (y1 - y0)
-    ix0, ix1, iy0, iy1 = max(x0, r), min(x1, W - r), max(y0, r), min(y1, H - r)
+    ix0, ix1, iy0, iy1 = max(x0, lo), min(x1, W - lo), max(y0, lo), min(y1, H - lo)
     interior = ix0 < ix1 and iy0 < iy1
     inner = (ix1 - ix0) * (iy1 - iy0) if interior else 0
     s.Cells += cells
@@ -63,12 +66,12 @@ def band_stats(s, r, W, H, nin, nout, b):
         s.KernelWritten += inner * 4 * nout
 
 
-def ref_stats(k, W, H, tw, th, chunked):
+def ref_stats(k, W, H, tw, th, chunked, rmin=None):
     s = engine.Stats()
     r, nin, nout = k.radius, k.nin, k.nout
     if not chunked:
         for b in plan_bands(W, H, tw, th):
-            band_stats(s, r, W, H, nin, nout, b)
+            band_stats(s, r, W, H, nin, nout, b, rmin)
     else:
         tileW = min(tw, W) if tw > 0 else W
         tileH = min(th, H) if th > 0 else H
@@ -79,7 +82,7 @@ def ref_stats(k, W, H, tw, th, chunked):
                 s.Tiles += 1
                 s.SourceRead += (rx1 - rx0) * (ry1 - ry0) * 4 * nin
                 for b in plan_bands(x1 - x0, y1 - y0, 0, 0):
-                    band_stats(s, r, W, H, nin, nout, (b[0] + x0, b[1] + y0, b[2] + x0, b[3] + y0))
+                    band_stats(s, r, W, H, nin, nout, (b[0] + x0, b[1] + y0, b[2] + x0, b[3] + y0), rmin)
                 s.SinkWritten += (x1 - x0) * (y1 - y0) * 4 * nout
     s.Ideal = s.Cells * 4 * (nin + nout)
     return s
@@ -114,6 +117,55 @@ def test_stats_known_values():
     s = PlanStats(k, 20000, 20000, Options(TileHeight=256), chunked=True)
     assert s.Tiles == 79 and s.Cells == 400_000_000
     assert s.KernelWritten == s.Cells * 4
+
+
+FEATURE_STACKS = [
+    [terrain.SlopeOptions(CellSize=30), terrain.AspectOptions(CellSize=30)],
+    [terrain.SlopeOptions(CellSize=30), terrain.SlopeOptions(CellSize=30, FitRadius=4),
+     terrain.CurvatureOptions(CellSize=30, FitRadius=2)],
+    [terrain.HillshadeOptions(CellSize=30, FitRadius=3, Azimuth=315, Altitude=45)],
+]
+
+
+@pytest.mark.parametrize("si", range(len(FEATURE_STACKS)))
+@pytest.mark.parametrize("W,H", [(1, 1), (5, 3), (17, 5), (300, 200), (70000, 3)])
+@pytest.mark.parametrize("tw,th", [(0, 0), (7, 5), (64, 1), (256, 256)])
+@pytest.mark.parametrize("chunked", [False, True])
+def test_fanout_plan_stats_match_reference_loops(si, W, H, tw, th, chunked):
+    """A Features pipeline (terrain/features.go:103-117) has the largest
+    stage radius; every output's ring is its own radius, so the kernel runs
+    outside the narrowest one (internal/exec/tile.go:136-184)."""
+    if W * H > 1 << 18 and (tw, th) in ((7, 5), (64, 1)):
+        pytest.skip("too many units for the Python restatement")
+    k = terrain.newFeatures(FEATURE_STACKS[si])
+    rmin = min(p.radius for p in k.parts)
+    got = PlanStats(k, W, H, Options(TileWidth=tw, TileHeight=th), chunked=chunked)
+    assert got == ref_stats(k, W, H, tw, th, chunked, rmin)
+
+
+def test_features_panics():
+    """terrain.newFeatures (terrain/features.go:106-113) and each op's own
+    option checks, before anything runs."""
+    with pytest.raises(terrain.TerrainPanic, match="^terrain: Features has no outputs"):
+        terrain.newFeatures([])
+    with pytest.raises(terrain.TerrainPanic, match="^terrain: Features output 1 has no Op"):
+        terrain.newFeatures([terrain.SlopeOptions(CellSize=1), None])
+    with pytest.raises(terrain.TerrainPanic, match="^terrain: Features output 0 has Op GradientOptions"):
+        terrain.newFeatures([terrain.GradientOptions(CellSize=1)])
+    with pytest.raises(terrain.TerrainPanic, match="^terrain: FitRadius"):
+        terrain.newFeatures([terrain.SlopeOptions(CellSize=1, FitRadius=9)])
+    with pytest.raises(EnginePanic, match="^engine: stage 0 takes 1 inputs and 2 outputs"):
+        engine.FanOut([terrain.gradientOp(terrain.GradientOptions(CellSize=1))])
+    k = terrain.newFeatures([terrain.SlopeOptions(CellSize=1), terrain.AspectOptions(CellSize=1, FitRadius=3)])
+    assert (k.radius, k.nin, k.nout) == (3, 1, 2)
+    def ras(w, h):
+        return NewFloat32(w, h, np.zeros(w * h, np.float32))
+
+    dem = ras(8, 8)
+    with pytest.raises(EnginePanic, match="kernel takes 1 inputs and 2 outputs, got 1 and 1"):
+        engine.check([ras(8, 8)], [dem], k, Options())
+    with pytest.raises(EnginePanic, match="dst\\[1\\] is 7×8"):
+        engine.check([ras(8, 8), ras(7, 8)], [dem], k, Options())
 
 
 # -- option resolution --------------------------------------------------------------
