@@ -184,18 +184,35 @@ def main() -> int:
         ops.append(("Surface (Dx, Dy, Slope, Aspect, Hillshade)",
                     terrain.newSurface([True] * 5, terrain.SurfaceOptions(CellSize=30, Azimuth=315, Altitude=45)),
                     [sink, MemorySink(out2)] + [MemorySink(e) for e in extra]))
-        suite["Slope"] = {"gcells_s": round(value, 2), "op_hbm_frac": round(op_bytes / (el / args.steps) / 1e9 /
-                                                                              HBM_PEAK_GBS, 4),
+        # terrain.Features over the whole suite (Slope, Aspect, Hillshade and
+        # the three curvatures): six outputs from one read of the DEM.
+        extra.append(Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                                   torch.zeros_like(valid), 0))
+        stack = [terrain.SlopeOptions(CellSize=30), terrain.AspectOptions(CellSize=30),
+                 terrain.HillshadeOptions(CellSize=30, Azimuth=315, Altitude=45, ZFactor=1),
+                 terrain.CurvatureOptions(CellSize=30, Type=terrain.CurvatureProfile),
+                 terrain.CurvatureOptions(CellSize=30, Type=terrain.CurvaturePlan),
+                 terrain.CurvatureOptions(CellSize=30, Type=terrain.CurvatureMean)]
+        ops.append(("Features (Slope, Aspect, Hillshade, Curvature profile, plan, mean)",
+                    terrain.newFeatures(stack), [sink, MemorySink(out2)] + [MemorySink(e) for e in extra]))
+        suite["Slope"] = {"gcells_s": round(value, 2), "ms": round(ms_step, 4),
+                          "op_hbm_frac": round(op_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                           "data_kernel_ms": round(kern_ms, 4), "validity_kernel_ms": round(vkern_ms, 4)}
         n_steps = max(5, args.steps // 2)
         for name, k, sinks in ops:
             e2, t2 = time_op(k, sinks, n_steps, 2)
             nout = len(sinks)
             b = (4 + 4 * nout + 0.125 * (1 + nout)) * cells
-            suite[name] = {"gcells_s": round(cells * n_steps / e2 / 1e9, 2),
+            suite[name] = {"gcells_s": round(cells * n_steps / e2 / 1e9, 2), "ms": round(e2 * 1e3 / n_steps, 4),
                            "op_hbm_frac": round(b / (e2 / n_steps) / 1e9 / HBM_PEAK_GBS, 4),
                            "data_kernel_ms": round(t2.data_ms / max(1, t2.data_launches), 4)}
             log(f"bench: {name} {suite[name]}")
+        # The six ops one call each against the one-pass stack.
+        sep = sum(suite[n]["ms"] for n in ("Slope", "Aspect", "Hillshade", "CurvatureProfile", "CurvaturePlan",
+                                           "CurvatureMean"))
+        fused = [v for n, v in suite.items() if n.startswith("Features")][0]["ms"]
+        suite["Features vs six calls"] = {"six_calls_ms": round(sep, 4), "features_ms": fused,
+                                          "speedup": round(sep / fused, 3)}
         del out2, extra
 
     traffic = None

@@ -412,7 +412,9 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                     const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
                     const float dy0 = (gl + g1) + (g0 + g0);
                     const float dy1 = (g0 + gr) + (g1 + g1);
-#pragma unroll
+                    // Rolled: one copy of each op's code however many outputs
+                    // (unrolled, the kernel outgrows the instruction cache).
+#pragma unroll 1
                     for (int o = 0; o < kMaxOutputs; ++o) {
                         if (o < fan->n) {
                             const float *k = fan->k[o];

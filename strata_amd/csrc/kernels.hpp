@@ -14,7 +14,7 @@ namespace strata {
 
 constexpr int kMaxTaps = 320;   // 289 correlate weights (r = 8) fit comfortably
 constexpr int kMaxRadius = 8;   // focal.MaxRadius (focal/focal.go) and terrain.MaxRadius
-constexpr int kMaxOutputs = 5;  // terrain.Surface's products
+constexpr int kMaxOutputs = 8;  // outputs of one launch: Surface's 5 products, or a fan-out sweep's ops
 
 // The output region [ox, ox+w) x [oy, oy+h) of a W x H raster.
 struct Region {

@@ -33,12 +33,21 @@ STACKS = {
               T.AspectOptions(CellSize=10, FitRadius=4, ZeroForFlat=True, Trigonometric=True),
               T.CurvatureOptions(CellSize=30, Type=T.CurvatureMean),
               T.HillshadeOptions(CellSize=30, FitRadius=1, Azimuth=90, Altitude=60)],
-    # Seven radius-1 ops: more than one sweep's worth of outputs.
+    # Seven radius-1 ops in one sweep.
     "horn7": [T.SlopeOptions(CellSize=30), T.SlopeOptions(CellSize=30, Units=T.SlopeRadians),
               T.SlopeOptions(CellSize=5, CellSizeY=7, Units=T.SlopePercent),
               T.AspectOptions(CellSize=30), T.AspectOptions(CellSize=2, Trigonometric=True, ZeroForFlat=True),
               T.HillshadeOptions(CellSize=1, ZFactor=3, Azimuth=-117.5, Altitude=23),
               T.CurvatureOptions(CellSize=30, Type=T.CurvaturePlan)],
+    # Ten: more than one sweep's worth of outputs (8 per sweep).
+    "horn10": [T.SlopeOptions(CellSize=30), T.SlopeOptions(CellSize=30, Units=T.SlopeRadians),
+               T.AspectOptions(CellSize=30), T.HillshadeOptions(CellSize=30, Azimuth=315, Altitude=45),
+               T.CurvatureOptions(CellSize=30, Type=T.CurvatureProfile),
+               T.CurvatureOptions(CellSize=30, Type=T.CurvaturePlan),
+               T.CurvatureOptions(CellSize=30, Type=T.CurvatureMean),
+               T.SlopeOptions(CellSize=7, Units=T.SlopePercent),
+               T.HillshadeOptions(CellSize=4, CellSizeY=3, Azimuth=10, Altitude=80),
+               T.AspectOptions(CellSize=30, FitRadius=2)],
     # One fit feature alone.
     "fit1": [T.CurvatureOptions(CellSize=30, FitRadius=3, Type=T.CurvaturePlan)],
 }
@@ -85,7 +94,7 @@ def test_features_match_standalone(gpu, stack, kind):
     assert_same_parents(host_copy(dbufs), want, f"Features {stack} {kind}")
 
 
-@pytest.mark.parametrize("stack", ["mixed", "horn7"])
+@pytest.mark.parametrize("stack", ["mixed", "horn10"])
 @pytest.mark.parametrize("tw,th", [(64, 16), (0, 7), (37, 0)])
 def test_features_tiled_stats(gpu, stack, tw, th):
     import torch
