@@ -266,6 +266,58 @@ int strata_plan_stats_fanout(const strata_kernel *parts, int nparts, int64_t wid
                              const strata_options *opts, int chunked, strata_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Reductions: exec.Reduce / exec.ReduceChunked with the reduce        */
+/* package's reducers                                                  */
+/* ------------------------------------------------------------------ */
+/*
+ * exec.Reduce (internal/exec/reduce.go:103) and exec.ReduceChunked
+ * (internal/exec/reducechunked.go:14) over one raster, with the fold of
+ * reduce.Count / MinMax / Sum / Stats (reduce/reduce.go:88-228,
+ * reduce/sum.go:96-114, internal/summary/summary.go:59-104). Only valid
+ * cells take part; data under an invalid cell is never read.
+ *
+ * The result is the combined partial, not the finished float64 values:
+ * the exact sums come back as integers (what the reference's bins hold,
+ * internal/accum/accum.go:49-76, carried into one number), so the caller
+ * finishes Sum, Mean and StdDev with its own exact arithmetic
+ * (internal/accum/result.go), as the reference does.
+ *
+ * `what` selects the work: COUNT alone folds only validity; MINMAX adds the
+ * extremes; SUM the exact sum; MOMENTS the sum and the sum of squares.
+ * Counts and the NaN/infinity tallies are always filled.
+ *
+ * Stats as the reference's fold loops: per band Cells, Bands and
+ * KernelRead (reduce.go:242-257), per chunked tile Tiles and SourceRead
+ * (reducechunked.go:186-205); Ideal = Cells*4. A cancelled call returns
+ * STRATA_CANCELLED and no value (reduce.go:88-101).
+ */
+#define STRATA_REDUCE_COUNT 0x1
+#define STRATA_REDUCE_MINMAX 0x2
+#define STRATA_REDUCE_SUM 0x4
+#define STRATA_REDUCE_MOMENTS 0x8
+
+typedef struct strata_fold {
+    int64_t count;         /* valid cells                                  */
+    int64_t nan;           /* valid NaNs                                   */
+    int64_t pos_inf;       /* valid +Inf                                   */
+    int64_t neg_inf;       /* valid -Inf                                   */
+    int32_t not_neg_zero;  /* 1 once a valid value other than -0 was seen  */
+    float min;             /* Go min / max of the valid cells (NaN         */
+    float max;             /* absorbing, -0 < +0); +Inf/-Inf when count 0  */
+    int32_t reserved;
+    /* Sum of the finite valid cells times 2^150, an exact integer:
+     * two's complement, 64-bit limbs, least significant first. */
+    uint64_t sum[6];
+    /* Sum of their squares times 2^300, likewise (MOMENTS only). */
+    uint64_t sumsq[10];
+} strata_fold;
+
+int strata_reduce(int what, const strata_raster *src, const strata_options *opts, strata_fold *out,
+                  strata_stats *stats);
+int strata_reduce_chunked(int what, const strata_source *src, const strata_options *opts, strata_fold *out,
+                          strata_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Launch tuning (no counterpart in the reference): shapes only, never  */
 /* results                                                              */
 /* ------------------------------------------------------------------ */

@@ -203,6 +203,38 @@ class ServeStats(ctypes.Structure):
     ]
 
 
+REDUCE_COUNT = 0x1
+REDUCE_MINMAX = 0x2
+REDUCE_SUM = 0x4
+REDUCE_MOMENTS = 0x8
+
+
+class Fold(ctypes.Structure):
+    """strata_fold: a reduction's combined partial, the exact sums as
+    two's complement 64-bit limbs, least significant first."""
+    _fields_ = [
+        ("count", c_int64),
+        ("nan", c_int64),
+        ("pos_inf", c_int64),
+        ("neg_inf", c_int64),
+        ("not_neg_zero", c_int32),
+        ("min", ctypes.c_float),
+        ("max", ctypes.c_float),
+        ("reserved", c_int32),
+        ("sum", ctypes.c_uint64 * 6),
+        ("sumsq", ctypes.c_uint64 * 10),
+    ]
+
+
+def limbs_to_int(limbs) -> int:
+    """The signed integer of two's complement 64-bit limbs."""
+    v = 0
+    for i, w in enumerate(limbs):
+        v |= int(w) << (64 * i)
+    n = 64 * len(limbs)
+    return v - (1 << n) if v >> (n - 1) else v
+
+
 # The symbols include/strata_hip.h declares, for the export check.
 EXPORTS = (
     "strata_init",
@@ -214,6 +246,8 @@ EXPORTS = (
     "strata_process_fanout",
     "strata_process_fanout_chunked",
     "strata_plan_stats_fanout",
+    "strata_reduce",
+    "strata_reduce_chunked",
     "strata_get_tuning",
     "strata_set_tuning",
     "strata_selfcheck",
@@ -274,6 +308,10 @@ def lib() -> ctypes.CDLL:
     L.strata_plan_stats_fanout.argtypes = [POINTER(Kernel), c_int, c_int64, c_int64, POINTER(Options), c_int,
                                            POINTER(Stats)]
     L.strata_plan_stats_fanout.restype = c_int
+    L.strata_reduce.argtypes = [c_int, POINTER(Raster), POINTER(Options), POINTER(Fold), POINTER(Stats)]
+    L.strata_reduce.restype = c_int
+    L.strata_reduce_chunked.argtypes = [c_int, POINTER(Source), POINTER(Options), POINTER(Fold), POINTER(Stats)]
+    L.strata_reduce_chunked.restype = c_int
     L.strata_get_tuning.argtypes = [POINTER(Tuning)]
     L.strata_get_tuning.restype = c_int
     L.strata_set_tuning.argtypes = [POINTER(Tuning)]

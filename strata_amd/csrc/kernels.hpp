@@ -102,6 +102,53 @@ hipError_t launch_correlate_error(const Params &P, const Region &rg, const SrcVi
 hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long long *dev3, hipStream_t st);
 
 // ---------------------------------------------------------------------------
+// Reductions: one raster folded to counts, extremes and exact sums
+// ---------------------------------------------------------------------------
+
+// Exact sums as integers binned by float32 exponent field (the layout of
+// internal/accum/accum.go:49-76): a finite value with field f adds its
+// signed significand to sum[max(f, 1)]; its squared significand (48 bits)
+// adds its low 24 bits to sq[2 max(f, 1)] and its high 24 bits to
+// sq[2 max(f, 1) + 24]. Bin k of sum weighs 2^(k - 150), bin j of sq
+// 2^(j - 300). Each bin grows by less than 2^24 per value, so int64 bins
+// hold 2^39 values.
+constexpr int kFoldSumBins = 256;
+constexpr int kFoldSqBins = 2 * 255 + 24 + 1;
+struct FoldBins {
+    long long sum[kFoldSumBins];
+    long long sq[kFoldSqBins];
+};
+
+// What is not binned: counts of the valid cells, of NaNs and of each
+// infinity; nnz nonzero once a valid value other than -0 was folded; the
+// Go min and max of the valid cells (+Inf and -Inf when there are none).
+struct FoldPart {
+    unsigned long long count, nan, pinf, ninf;
+    unsigned nnz;
+    float mn, mx;
+    unsigned pad;
+};
+
+// One raster region to fold: cell (x, y) at p[y*stride + x] for x < w,
+// y < h, valid when bits is null or bit off + y*mstride + x is set.
+struct FoldArgs {
+    const float *p;
+    int64_t stride, w, h;
+    const uint64_t *bits;
+    int64_t off, mstride;
+    int32_t what;  // STRATA_REDUCE_* flags
+    int32_t pad;
+};
+
+constexpr int kFoldMaxBlocks = 2048;  // partials one launch writes at most
+
+// Folds a's valid cells into bins (atomically) and the running total,
+// using parts (kFoldMaxBlocks entries) as scratch; clear: start bins and
+// total from empty first.
+hipError_t launch_fold(const FoldArgs &a, FoldBins *bins, FoldPart *parts, FoldPart *total, bool clear,
+                       hipStream_t st);
+
+// ---------------------------------------------------------------------------
 // Batches: many tiles, mixed ops, one persistent launch over a work list
 // ---------------------------------------------------------------------------
 

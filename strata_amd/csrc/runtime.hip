@@ -1023,9 +1023,271 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     return status;
 }
 
+// ---------------------------------------------------------------------------
+// Reductions (internal/exec/reduce.go, reducechunked.go)
+// ---------------------------------------------------------------------------
+
+// Adds v * 2^k to the n-limb two's complement integer at limbs.
+void add_shifted(uint64_t *limbs, int n, long long v, int k)
+{
+    const int w = k >> 6, sh = k & 63;
+    const unsigned __int128 wide = (unsigned __int128)(__int128)v << sh;
+    const uint64_t lo = (uint64_t)wide, hi = (uint64_t)(wide >> 64);
+    const uint64_t ext = v < 0 ? ~0ull : 0ull;
+    unsigned __int128 carry = 0;
+    for (int j = w; j < n; ++j) {
+        const uint64_t add = j == w ? lo : (j == w + 1 ? hi : ext);
+        const unsigned __int128 t = (unsigned __int128)limbs[j] + add + carry;
+        limbs[j] = (uint64_t)t;
+        carry = t >> 64;
+    }
+}
+
+// The device's fold of one call, read back: the exact integers carried
+// out of the bins (internal/accum/result.go exact) and the tallies.
+void fold_result(const FoldBins &b, const FoldPart &t, int what, strata_fold *out)
+{
+    memset(out, 0, sizeof *out);
+    out->count = (int64_t)t.count;
+    out->nan = (int64_t)t.nan;
+    out->pos_inf = (int64_t)t.pinf;
+    out->neg_inf = (int64_t)t.ninf;
+    out->not_neg_zero = t.nnz != 0;
+    out->min = t.mn;
+    out->max = t.mx;
+    if (what & (STRATA_REDUCE_SUM | STRATA_REDUCE_MOMENTS))
+        for (int k = 0; k < kFoldSumBins; ++k)
+            if (b.sum[k]) add_shifted(out->sum, 6, b.sum[k], k);
+    if (what & STRATA_REDUCE_MOMENTS)
+        for (int j = 0; j < kFoldSqBins; ++j)
+            if (b.sq[j]) add_shifted(out->sumsq, 10, b.sq[j], j);
+}
+
+// Device scratch of one reduction: the bins, the total and the partials.
+struct FoldScratch {
+    char *dev = nullptr;
+    hipStream_t st = nullptr;
+    FoldBins *bins() const { return (FoldBins *)dev; }
+    FoldPart *total() const { return (FoldPart *)(dev + sizeof(FoldBins)); }
+    FoldPart *parts() const { return (FoldPart *)(dev + sizeof(FoldBins) + 256); }
+    static size_t bytes() { return sizeof(FoldBins) + 256 + sizeof(FoldPart) * kFoldMaxBlocks; }
+    ~FoldScratch()
+    {
+        if (dev) (void)hipFreeAsync(dev, st);
+    }
+};
+
+int check_reduce(int what, const strata_options *opts)
+{
+    if (what == 0 || (what & ~(STRATA_REDUCE_COUNT | STRATA_REDUCE_MINMAX | STRATA_REDUCE_SUM |
+                               STRATA_REDUCE_MOMENTS)))
+        return fail(STRATA_EINVAL, "engine: unknown reduction %#x", what);
+    if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
+        return fail(STRATA_EINVAL, "engine: negative Options {TileWidth:%lld TileHeight:%lld Workers:%lld}",
+                    (long long)opts->tile_width, (long long)opts->tile_height, (long long)opts->workers);
+    return STRATA_OK;
+}
+
+FoldArgs fold_args(const strata_raster &r, int what, bool masked)
+{
+    FoldArgs a{};
+    a.p = r.data;
+    a.stride = r.stride;
+    a.w = r.width;
+    a.h = r.height;
+    a.bits = masked ? r.valid : nullptr;
+    a.off = r.valid_offset;
+    a.mstride = r.stride;
+    a.what = what;
+    return a;
+}
+
+// Reads the scratch back into out (waits for the stream).
+int fold_read(const FoldScratch &fs, int what, strata_fold *out)
+{
+    FoldBins b;
+    FoldPart t;
+    HIP_TRY(hipMemcpyAsync(&b, fs.bins(), sizeof b, hipMemcpyDeviceToHost, fs.st), "reading the bins");
+    HIP_TRY(hipMemcpyAsync(&t, fs.total(), sizeof t, hipMemcpyDeviceToHost, fs.st), "reading the fold");
+    HIP_TRY(hipStreamSynchronize(fs.st), "synchronizing");
+    if (out) fold_result(b, t, what, out);
+    return STRATA_OK;
+}
+
+// A band's counters of a fold (reduce.go:242-257): no outputs, no halo.
+void fold_band_stats(strata_stats &s, const Rect &b)
+{
+    const int64_t cells = (b.x1 - b.x0) * (b.y1 - b.y0);
+    s.cells += cells;
+    s.bands += 1;
+    s.kernel_read += cells * kBytesPerCell;
+}
+
 } // namespace
 
 extern "C" {
+
+int strata_reduce(int what, const strata_raster *src, const strata_options *opts, strata_fold *out,
+                  strata_stats *stats)
+{
+    int rc = check_reduce(what, opts);
+    if (rc) return rc;
+    if (!src) return fail(STRATA_EINVAL, "engine: reducer takes 1 inputs, got 0");
+    if ((rc = check_raster("src", 0, *src))) return rc;
+    if (out) memset(out, 0, sizeof *out);
+    // runWorkers checks the context before every band (internal/exec/
+    // worker.go:144-181); the device folds the whole raster in one launch,
+    // so the check comes once, before it.
+    if (cancelled(opts)) return STRATA_CANCELLED;
+    if ((rc = ensure_device())) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    Staged staged;
+    const strata_raster *d = src;
+    if (src->mem == STRATA_MEM_HOST) {
+        if ((rc = stage_in(*src, staged, true, st))) return rc;
+        d = &staged.dev;
+    }
+    FoldScratch fs;
+    fs.st = st;
+    HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+    HIP_TRY(launch_fold(fold_args(*d, what, d->valid != nullptr), fs.bins(), fs.parts(), fs.total(), true, st),
+            "launching the fold");
+    if ((rc = fold_read(fs, what, out))) return rc;
+    if (stats) {
+        strata_stats s{};
+        const Plan p = new_plan(src->width, src->height, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
+        for (int64_t b = 0; b < p.bands; ++b) fold_band_stats(s, plan_band(p, b));
+        add_stats(stats, s, 1, 0);
+    }
+    g_err.clear();
+    return STRATA_OK;
+}
+
+int strata_reduce_chunked(int what, const strata_source *src, const strata_options *opts, strata_fold *out,
+                          strata_stats *stats)
+{
+    int rc = check_reduce(what, opts);
+    if (rc) return rc;
+    if (!src || (!src->resident && !src->read)) return fail(STRATA_EINVAL, "engine: src[0] is a nil source");
+    const int64_t W = src->width, H = src->height;
+    if (W <= 0 || H <= 0)
+        return fail(STRATA_EINVAL, "engine: src[0] is %lld×%lld; sizes must be positive", (long long)W,
+                    (long long)H);
+    if (src->resident) {
+        if ((rc = check_raster("src", 0, *src->resident))) return rc;
+        if (src->resident->width != W || src->resident->height != H)
+            return fail(STRATA_EINVAL, "engine: src[0]'s raster does not match its size");
+        if (src->resident->mem == STRATA_MEM_HOST)
+            return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+    }
+    if (out) memset(out, 0, sizeof *out);
+    if ((rc = ensure_device())) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    const TileGrid g = tile_grid(W, H, opts);
+    strata_stats s{};
+    auto tile_counts = [&](const Rect &t) {
+        s.tiles += 1;
+        s.source_read += (t.x1 - t.x0) * (t.y1 - t.y0) * kBytesPerCell;
+    };
+    auto tile_bands = [&](const Rect &t) {
+        const Plan p = new_plan(t.x1 - t.x0, t.y1 - t.y0, 0, 0);
+        for (int64_t b = 0; b < p.bands; ++b) {
+            Rect band = plan_band(p, b);
+            band.x0 += t.x0;
+            band.x1 += t.x0;
+            band.y0 += t.y0;
+            band.y1 += t.y0;
+            fold_band_stats(s, band);
+        }
+    };
+    FoldScratch fs;
+    fs.st = st;
+    int status = STRATA_OK;
+    if (src->resident) {
+        // Read in place: one launch over the raster, the tiles' counters in
+        // closed form.
+        if (cancelled(opts)) return STRATA_CANCELLED;
+        HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+        HIP_TRY(launch_fold(fold_args(*src->resident, what, src->masked && src->resident->valid), fs.bins(),
+                            fs.parts(), fs.total(), true, st),
+                "launching the fold");
+        if ((rc = fold_read(fs, what, out))) return rc;
+        for (int64_t t = 0; t < g.tiles; ++t) {
+            const Rect tr = tile_rect(g, t);
+            tile_counts(tr);
+            tile_bands(tr);
+        }
+    } else {
+        // Tiles read by the callback into a pinned buffer, copied to the
+        // device and folded into the same bins (reducechunked.go:186-205).
+        const bool masked = src->masked != 0;
+        const int64_t stride = masked ? ((g.tileW + 63) & ~(int64_t)63) : g.tileW;
+        const int64_t cells = (g.tileH - 1) * stride + g.tileW;
+        const int64_t words = masked ? cdiv(cells, 64) : 0;
+        float *hdata = nullptr, *ddata = nullptr;
+        uint64_t *hmask = nullptr, *dmask = nullptr;
+        struct Bufs {
+            float *&hd, *&dd;
+            uint64_t *&hm, *&dm;
+            ~Bufs()
+            {
+                if (hd) (void)hipHostFree(hd);
+                if (hm) (void)hipHostFree(hm);
+                if (dd) (void)hipFree(dd);
+                if (dm) (void)hipFree(dm);
+            }
+        } bufs{hdata, ddata, hmask, dmask};
+        HIP_TRY(hipHostMalloc((void **)&hdata, (size_t)cells * 4, hipHostMallocDefault), "pinned buffer");
+        HIP_TRY(hipMalloc((void **)&ddata, (size_t)cells * 4), "tile buffer");
+        if (masked) {
+            HIP_TRY(hipHostMalloc((void **)&hmask, (size_t)words * 8, hipHostMallocDefault), "pinned mask");
+            HIP_TRY(hipMalloc((void **)&dmask, (size_t)words * 8), "tile mask");
+        }
+        HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+        bool first = true;
+        for (int64_t t = 0; t < g.tiles; ++t) {
+            if (cancelled(opts)) {
+                status = STRATA_CANCELLED;
+                break;
+            }
+            const Rect tr = tile_rect(g, t);
+            const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
+            tile_counts(tr);
+            if (hmask) memset(hmask, 0, (size_t)words * 8);
+            if (src->read(src->user, hdata, tw, th, stride, hmask, 0, tr.x0, tr.y0) != 0) {
+                status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error", (long long)tr.x0,
+                              (long long)tr.y0);
+                break;
+            }
+            const int64_t n = (th - 1) * stride + tw;
+            HIP_TRY(hipMemcpyAsync(ddata, hdata, (size_t)n * 4, hipMemcpyHostToDevice, st), "tile in");
+            if (hmask)
+                HIP_TRY(hipMemcpyAsync(dmask, hmask, (size_t)cdiv(n, 64) * 8, hipMemcpyHostToDevice, st),
+                        "tile mask in");
+            strata_raster tv{};
+            tv.data = ddata;
+            tv.width = tw;
+            tv.height = th;
+            tv.stride = stride;
+            tv.valid = dmask;
+            tv.valid_offset = 0;
+            HIP_TRY(launch_fold(fold_args(tv, what, masked), fs.bins(), fs.parts(), fs.total(), first, st),
+                    "launching the fold");
+            first = false;
+            // The pinned buffer is reused by the next tile.
+            HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
+            tile_bands(tr);
+        }
+        if (status == STRATA_OK && (rc = fold_read(fs, what, out))) return rc;
+    }
+    add_stats(stats, s, 1, 0);
+    if (status != STRATA_OK) {
+        if (out) memset(out, 0, sizeof *out);
+        return status;
+    }
+    g_err.clear();
+    return STRATA_OK;
+}
 
 int strata_plan_stats(const strata_kernel *k, int ndst, int nsrc, int64_t width, int64_t height,
                       const strata_options *opts, int chunked, strata_stats *stats)

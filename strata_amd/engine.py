@@ -506,6 +506,32 @@ def _resident(x) -> Optional[Float32Raster]:
     return None
 
 
+def _read_callback(ctx, s: RasterSource, j: int, errors: List[Exception]):
+    """The read callback of source j: ReadWindow into the library's pinned
+    buffer; an error or a panic is kept in errors and reported as a
+    nonzero return."""
+    def read(user, buf, ww, hh, stride, valid, voff, x, y):
+        try:
+            n = (hh - 1) * stride + ww
+            data = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_float)), shape=(n,))
+            vm = None
+            if valid:
+                vm = np.ctypeslib.as_array(ctypes.cast(valid, ctypes.POINTER(ctypes.c_uint64)),
+                                           shape=((voff + n + 63) >> 6,))
+            v = Float32Raster(data, ww, hh, stride, vm, voff)
+            err = s.ReadWindow(ctx, v, x, y)
+            if err is not None:
+                errors.append(SourceError(f"engine: reading src[{j}] at ({x}, {y}): {err}"))
+                return 1
+            if vm is not None and not s.Masked():
+                _copy_validity(v, Float32Raster(data, ww, hh, stride, None, 0))
+            return 0
+        except Exception as e:  # a panicking source
+            errors.append(e)
+            return 1
+    return _lib.READ_FN(read)
+
+
 def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[RasterSource], k: KernelDesc,
                    opts: Optional[Options] = None, *, async_: bool = False) -> Optional[Exception]:
     """exec.ProcessChunked: tiles read with their halo, computed on the
@@ -552,26 +578,7 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
     errors: List[Exception] = []
 
     def make_read(s: RasterSource, j: int):
-        def read(user, buf, ww, hh, stride, valid, voff, x, y):
-            try:
-                n = (hh - 1) * stride + ww
-                data = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_float)), shape=(n,))
-                vm = None
-                if valid:
-                    vm = np.ctypeslib.as_array(ctypes.cast(valid, ctypes.POINTER(ctypes.c_uint64)),
-                                               shape=((voff + n + 63) >> 6,))
-                v = Float32Raster(data, ww, hh, stride, vm, voff)
-                err = s.ReadWindow(ctx, v, x, y)
-                if err is not None:
-                    errors.append(SourceError(f"engine: reading src[{j}] at ({x}, {y}): {err}"))
-                    return 1
-                if vm is not None and not s.Masked():
-                    _copy_validity(v, Float32Raster(data, ww, hh, stride, None, 0))
-                return 0
-            except Exception as e:  # a panicking source
-                errors.append(e)
-                return 1
-        return _lib.READ_FN(read)
+        return _read_callback(ctx, s, j, errors)
 
     def make_write(d: RasterSink, j: int):
         def write(user, buf, ww, hh, stride, valid, voff, x, y):
@@ -655,3 +662,81 @@ def _words_meet(a: Float32Raster, b: Float32Raster) -> bool:
     a0, an = words(a)
     b0, bn = words(b)
     return _spans_meet(a0, an, b0, bn)
+
+
+# -- reductions (internal/exec/reduce.go, reducechunked.go) ------------------------------
+
+def Reduce(ctx: Optional[Context], src: List[Float32Raster], what: int,
+           opts: Optional[Options] = None):
+    """exec.Reduce over one raster with the device fold selected by what
+    (_lib.REDUCE_*): (the combined partial as a _lib.Fold, None), or
+    (None, Canceled()) when ctx was done before the fold ran."""
+    opts = opts or Options()
+    if len(src) != 1:
+        raise EnginePanic(f"engine: reducer takes 1 inputs, got {len(src)}")
+    if opts.TileWidth < 0 or opts.TileHeight < 0 or opts.Workers < 0:
+        raise EnginePanic(f"engine: negative Options {opts}")
+    err = src[0].Validate()
+    if err is not None:
+        raise EnginePanic(f"engine: src[0]: {err}")
+    cs = src[0].to_c()
+    co = _c_options(opts, ctx, _stream_handle(src))
+    out = _lib.Fold()
+    st = _lib.Stats()
+    rc = _lib.lib().strata_reduce(what, ctypes.byref(cs), ctypes.byref(co), ctypes.byref(out), ctypes.byref(st))
+    if rc < 0:
+        _raise_status(rc)
+    if opts.Stats is not None:
+        opts.Stats._add_c(st)
+    if rc == _lib.CANCELLED:
+        return None, Canceled()
+    return out, None
+
+
+def ReduceChunked(ctx: Optional[Context], src: List[RasterSource], what: int,
+                  opts: Optional[Options] = None):
+    """exec.ReduceChunked over one source: tiles read by the source's
+    ReadWindow (a memory source in device memory is read in place) and
+    folded on the device. (fold, None), or (None, err) for cancellation or
+    a source's error."""
+    opts = opts or Options()
+    if len(src) != 1:
+        raise EnginePanic(f"engine: reducer takes 1 inputs, got {len(src)}")
+    if opts.TileWidth < 0 or opts.TileHeight < 0 or opts.Workers < 0:
+        raise EnginePanic(f"engine: negative Options {opts}")
+    s = src[0]
+    if s is None:
+        raise EnginePanic("engine: src[0] is a nil source")
+    w, h = s.Size()
+    if w <= 0 or h <= 0:
+        raise EnginePanic(f"engine: src[0] is {w}×{h}; sizes must be positive")
+    errors: List[Exception] = []
+    keep = []
+    c = _lib.Source()
+    c.width, c.height, c.masked = w, h, 1 if s.Masked() else 0
+    res = _resident(s)
+    if res is not None:
+        rc_ = res.to_c()
+        keep.append(rc_)
+        c.resident = ctypes.pointer(rc_)
+    else:
+        fn = _read_callback(ctx, s, 0, errors)
+        keep.append(fn)
+        c.read = fn
+    co = _c_options(opts, ctx, _stream_handle([res] if res is not None else []))
+    out = _lib.Fold()
+    st = _lib.Stats()
+    rc = _lib.lib().strata_reduce_chunked(what, ctypes.byref(c), ctypes.byref(co), ctypes.byref(out),
+                                          ctypes.byref(st))
+    del keep
+    if opts.Stats is not None:
+        opts.Stats._add_c(st)
+    if errors and not isinstance(errors[0], SourceError):
+        raise errors[0]
+    if rc == _lib.EIO:
+        return None, (errors[0] if errors else SourceError(_lib.last_error()))
+    if rc < 0:
+        _raise_status(rc)
+    if rc == _lib.CANCELLED:
+        return None, Canceled()
+    return out, None
