@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--focal-size", type=int, default=16384,
                     help="side of the focal-suite raster (0: skip the focal suite)")
     ap.add_argument("--focal-steps", type=int, default=5, help="timed runs of each focal op")
+    ap.add_argument("--reduce-steps", type=int, default=10,
+                    help="timed runs of each reduction over the DEM (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -219,6 +221,10 @@ def main() -> int:
     if not args.no_traffic:
         traffic = pmc_traffic(args, N, log)
 
+    reduce_res = None
+    if args.reduce_steps > 0:
+        reduce_res = reduce_bench(args, dem, N, log)
+
     batch = None
     if args.batch_tiles > 0:
         del out, sink
@@ -272,6 +278,7 @@ def main() -> int:
         "batch": batch,
         "serving": serving,
         "focal": focal_res,
+        "reduce": reduce_res,
     }
     if serving and serving["levels"]:
         at = min(serving["levels"], key=lambda lv: abs(lv["qps"] - 1000.0))
@@ -280,6 +287,38 @@ def main() -> int:
                                    "arrival to completion"}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def reduce_bench(args, dem, N, log):
+    """reduce.Count / MinMax / Sum / Stats over the device-resident DEM: one
+    fold launch each (plus the one-block combine and the read-back of the
+    bins); whole-call wall time, and the HBM share of the bytes a fold must
+    read (4 B data + 1/8 B mask per cell; the mask alone for Count)."""
+    import torch
+
+    from strata_amd import _lib, engine
+    from strata_amd import reduce as rd
+
+    cells = N * N
+    res = {}
+    for name, what in (("Count", _lib.REDUCE_COUNT), ("MinMax", _lib.REDUCE_MINMAX), ("Sum", _lib.REDUCE_SUM),
+                       ("Stats", _lib.REDUCE_MINMAX | _lib.REDUCE_MOMENTS)):
+        for _ in range(2):
+            engine.Reduce(None, [dem], what)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.reduce_steps):
+            f, err = engine.Reduce(None, [dem], what)
+            assert err is None
+        el = (time.perf_counter() - t) / args.reduce_steps
+        b = (0.125 if what == _lib.REDUCE_COUNT else 4.125) * cells
+        res[name] = {"gcells_s": round(cells / el / 1e9, 2), "ms": round(el * 1e3, 4),
+                     "hbm_frac": round(b / el / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"bench: reduce {name} {res[name]}")
+    s = rd.Stats(dem)
+    res["summary"] = {"count": s.Count, "sum": s.Sum, "mean": s.Mean, "stddev": s.StdDev, "min": float(s.Min),
+                      "max": float(s.Max)}
+    return res
 
 
 def pmc_child(args) -> int:

@@ -282,9 +282,10 @@ int strata_plan_stats_fanout(const strata_kernel *parts, int nparts, int64_t wid
  * finishes Sum, Mean and StdDev with its own exact arithmetic
  * (internal/accum/result.go), as the reference does.
  *
- * `what` selects the work: COUNT alone folds only validity; MINMAX adds the
- * extremes; SUM the exact sum; MOMENTS the sum and the sum of squares.
- * Counts and the NaN/infinity tallies are always filled.
+ * `what` selects the work: COUNT alone folds only validity (no cell is
+ * read; only count is filled); MINMAX adds the extremes and the NaN and
+ * infinity tallies; SUM the exact sum; MOMENTS the sum and the sum of
+ * squares.
  *
  * Stats as the reference's fold loops: per band Cells, Bands and
  * KernelRead (reduce.go:242-257), per chunked tile Tiles and SourceRead

@@ -106,14 +106,15 @@ hipError_t launch_selfcheck(int kind, uint64_t lo, uint64_t n, unsigned long lon
 // ---------------------------------------------------------------------------
 
 // Exact sums as integers binned by float32 exponent field (the layout of
-// internal/accum/accum.go:49-76): a finite value with field f adds its
-// signed significand to sum[max(f, 1)]; its squared significand (48 bits)
-// adds its low 24 bits to sq[2 max(f, 1)] and its high 24 bits to
-// sq[2 max(f, 1) + 24]. Bin k of sum weighs 2^(k - 150), bin j of sq
-// 2^(j - 300). Each bin grows by less than 2^24 per value, so int64 bins
-// hold 2^39 values.
-constexpr int kFoldSumBins = 256;
-constexpr int kFoldSqBins = 2 * 255 + 24 + 1;
+// internal/accum/accum.go:49-76): a finite value with field f (e = max(f,
+// 1)) contributes its signed significand at weight 2^e, and its squared
+// significand's low and high 24 bits at weights 2^(2e) and 2^(2e + 24).
+// Bin k of sum weighs 2^(k - 150), bin j of sq 2^(j - 300). A bin may also
+// take a partial sum at a lower weight, split into 32-bit pieces at bins k
+// and k + 32, which is what the 48 bins above the exponents are for; no
+// bin grows by 2^32 per 512 values, so int64 bins hold 2^40 values.
+constexpr int kFoldSumBins = 256 + 48;
+constexpr int kFoldSqBins = 2 * 255 + 24 + 48 + 1;
 struct FoldBins {
     long long sum[kFoldSumBins];
     long long sq[kFoldSqBins];

@@ -64,6 +64,8 @@ def oracle_of(r: Float32Raster):
 
 def check_fold(f: _lib.Fold, want: dict, what=_lib.REDUCE_MINMAX | _lib.REDUCE_MOMENTS):
     assert f.count == want["count"]
+    if what == _lib.REDUCE_COUNT:
+        return  # validity alone: no cell is read
     assert (f.nan, f.pos_inf, f.neg_inf) == (want["nan"], want["pos_inf"], want["neg_inf"])
     assert bool(f.not_neg_zero) == want["not_neg_zero"]
     if want["count"]:
