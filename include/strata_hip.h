@@ -203,22 +203,39 @@ typedef int (*strata_write_fn)(void *user, const float *buf, int64_t w, int64_t 
 /* A source is either resident (a raster the library reads windows of
  * directly, MemorySource over device memory: engine/source.go:76-107) or
  * a callback reader. */
+/* engine.RawSource / RawSink (engine/raw.go:28-245) over a file the
+ * library reads and writes itself: width*height little-endian float32
+ * cells, row-major, from byte `offset`. With has_fill a source is Masked
+ * and a cell equal to fill as a float (-0 == 0; a NaN fill matches any
+ * NaN) is invalid, its validity computed on the device as tiles arrive; a
+ * sink with has_fill gets fill in every invalid cell (engine/raw.go:
+ * 247-273). Short reads and failed writes are STRATA_EIO. */
+typedef struct strata_raw {
+    int32_t fd;       /* open file descriptor; pread/pwrite only        */
+    int32_t has_fill;
+    float fill;
+    int32_t reserved;
+    int64_t offset;   /* byte offset of cell (0, 0)                     */
+} strata_raw;
+
 typedef struct strata_source {
     int64_t width, height;
-    int32_t masked;          /* Masked() */
+    int32_t masked;          /* Masked() (a raw source: has_fill)        */
     int32_t reserved;
-    strata_read_fn read;     /* used when resident is NULL */
+    strata_read_fn read;     /* used when resident and raw are NULL      */
     void *user;
     const strata_raster *resident;
+    const strata_raw *raw;   /* used when resident is NULL               */
 } strata_source;
 
 typedef struct strata_sink {
     int64_t width, height;
     int32_t masked;
     int32_t reserved;
-    strata_write_fn write;   /* used when resident is NULL */
+    strata_write_fn write;   /* used when resident and raw are NULL      */
     void *user;
     const strata_raster *resident;
+    const strata_raw *raw;   /* used when resident is NULL               */
 } strata_sink;
 
 /*

@@ -112,6 +112,17 @@ WRITE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64
                             c_int64)
 
 
+class Raw(ctypes.Structure):
+    """strata_raw: a raw float32 file the library reads or writes itself."""
+    _fields_ = [
+        ("fd", c_int32),
+        ("has_fill", c_int32),
+        ("fill", ctypes.c_float),
+        ("reserved", c_int32),
+        ("offset", c_int64),
+    ]
+
+
 class Source(ctypes.Structure):
     _fields_ = [
         ("width", c_int64),
@@ -121,6 +132,7 @@ class Source(ctypes.Structure):
         ("read", READ_FN),
         ("user", c_void_p),
         ("resident", POINTER(Raster)),
+        ("raw", POINTER(Raw)),
     ]
 
 
@@ -133,6 +145,7 @@ class Sink(ctypes.Structure):
         ("write", WRITE_FN),
         ("user", c_void_p),
         ("resident", POINTER(Raster)),
+        ("raw", POINTER(Raw)),
     ]
 
 

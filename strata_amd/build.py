@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstrata_hip.so")
-SOURCES = ["kernels.hip", "runtime.hip", "batch.hip", "serve.hip", "reduce.hip"]
+SOURCES = ["kernels.hip", "runtime.hip", "batch.hip", "serve.hip", "reduce.hip", "rawio.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -150,6 +150,24 @@ hipError_t launch_fold(const FoldArgs &a, FoldBins *bins, FoldPart *parts, FoldP
                        hipStream_t st);
 
 // ---------------------------------------------------------------------------
+// Raw files: fill-value NoData of a tile buffer
+// ---------------------------------------------------------------------------
+
+// A chunked tile buffer: cell (x, y) at p[y*stride + x], its validity bit
+// y*stride + x of bits; stride is a multiple of 64.
+struct RawTile {
+    float *p;
+    int64_t stride, w, h;
+    uint64_t *bits;
+};
+
+// Writes the validity words of the tile's rows from the fill value: a cell
+// is invalid where it equals fill as a float (a NaN fill: where it is a NaN).
+hipError_t launch_raw_validity(const RawTile &t, float fill, hipStream_t st);
+// Stores fill into every invalid cell of the tile.
+hipError_t launch_raw_fill(const RawTile &t, float fill, hipStream_t st);
+
+// ---------------------------------------------------------------------------
 // Batches: many tiles, mixed ops, one persistent launch over a work list
 // ---------------------------------------------------------------------------
 

@@ -10,8 +10,10 @@
 //   cancellation     internal/exec/worker.go:112-181 (claims in order,
 //                    a claimed unit always finishes: a prefix is written)
 #include <hip/hip_runtime.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -715,6 +717,67 @@ int strata_timing_end(double *data_ms, int64_t *data_launches, double *validity_
 
 namespace {
 
+// ---------------------------------------------------------------------------
+// Raw files (engine/raw.go)
+// ---------------------------------------------------------------------------
+
+// Reads rows y..y+h of columns x..x+w of a W-wide raw file into buf (rows
+// `stride` floats apart). Errors as the reference's wrap of a raw source's
+// error (internal/exec/chunked.go:379-393, engine/raw.go:86-92).
+int raw_read(const strata_raw &r, int64_t W, float *buf, int64_t w, int64_t h, int64_t stride, int64_t x, int64_t y,
+             const char *what)
+{
+    for (int64_t row = 0; row < h; ++row) {
+        char *p = (char *)(buf + row * stride);
+        size_t left = (size_t)w * 4;
+        off_t off = (off_t)(r.offset + 4 * ((y + row) * W + x));
+        while (left > 0) {
+            const ssize_t n = pread(r.fd, p, left, off);
+            if (n < 0 && errno == EINTR) continue;
+            if (n <= 0)
+                return fail(STRATA_EIO, "engine: reading %s at (%lld, %lld): engine: raw source: reading row %lld: %s",
+                            what, (long long)x, (long long)y, (long long)(y + row),
+                            n == 0 ? "unexpected EOF" : strerror(errno));
+            p += n;
+            left -= (size_t)n;
+            off += n;
+        }
+    }
+    return STRATA_OK;
+}
+
+// Writes rows of buf into the W-wide raw file at (x, y).
+int raw_write(const strata_raw &r, int64_t W, const float *buf, int64_t w, int64_t h, int64_t stride, int64_t x,
+              int64_t y, int o)
+{
+    for (int64_t row = 0; row < h; ++row) {
+        const char *p = (const char *)(buf + row * stride);
+        size_t left = (size_t)w * 4;
+        off_t off = (off_t)(r.offset + 4 * ((y + row) * W + x));
+        while (left > 0) {
+            const ssize_t n = pwrite(r.fd, p, left, off);
+            if (n < 0 && errno == EINTR) continue;
+            if (n <= 0)
+                return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): engine: raw sink: writing row %lld: %s",
+                            o, (long long)x, (long long)y, (long long)(y + row),
+                            n == 0 ? "short write" : strerror(errno));
+            p += n;
+            left -= (size_t)n;
+            off += n;
+        }
+    }
+    return STRATA_OK;
+}
+
+int check_raw(const strata_raw *r, int32_t masked, const char *name, int i)
+{
+    if (r->fd < 0) return fail(STRATA_EINVAL, "engine: %s[%d]: nil file", name, i);
+    if ((r->has_fill != 0) != (masked != 0))
+        return fail(STRATA_EINVAL, "engine: %s[%d]: a raw file is Masked exactly when it has a fill value", name, i);
+    if (r->offset < 0) return fail(STRATA_EINVAL, "engine: %s[%d]: negative file offset", name, i);
+    return STRATA_OK;
+}
+
 int plan_stats(const Job &J, int nsrc, int64_t width, int64_t height, const strata_options *opts, int chunked,
                strata_stats *stats)
 {
@@ -823,7 +886,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         if (src[i].width != W || src[i].height != H)
             return fail(STRATA_EINVAL, "engine: src[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
                         (long long)src[i].width, (long long)src[i].height, (long long)W, (long long)H);
-        if (!src[i].resident && !src[i].read) return fail(STRATA_EINVAL, "engine: src[%d] is a nil source", i);
+        if (!src[i].resident && !src[i].raw && !src[i].read)
+            return fail(STRATA_EINVAL, "engine: src[%d] is a nil source", i);
+        if (!src[i].resident && src[i].raw && (rc = check_raw(src[i].raw, src[i].masked, "src", i))) return rc;
         if (src[i].resident) {
             if ((rc = check_raster("src", i, *src[i].resident))) return rc;
             if (src[i].resident->width != W || src[i].resident->height != H)
@@ -836,7 +901,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         if (dst[i].width != W || dst[i].height != H)
             return fail(STRATA_EINVAL, "engine: dst[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
                         (long long)dst[i].width, (long long)dst[i].height, (long long)W, (long long)H);
-        if (!dst[i].resident && !dst[i].write) return fail(STRATA_EINVAL, "engine: dst[%d] is a nil sink", i);
+        if (!dst[i].resident && !dst[i].raw && !dst[i].write)
+            return fail(STRATA_EINVAL, "engine: dst[%d] is a nil sink", i);
+        if (!dst[i].resident && dst[i].raw && (rc = check_raw(dst[i].raw, dst[i].masked, "dst", i))) return rc;
         if (dst[i].resident && (rc = check_raster("dst", i, *dst[i].resident))) return rc;
         if (masked && !dst[i].masked)
             return fail(STRATA_EINVAL,
@@ -942,17 +1009,30 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             } else {
                 Buf &b = bufs[0];
                 const int64_t ww = rx1 - rx0, hh = ry1 - ry0;
-                if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
-                if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
-                    status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
-                                  (long long)rx0, (long long)ry0);
-                    break;
-                }
                 const int64_t cells = (hh - 1) * b.stride + ww;
-                HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st), "tile in");
-                if (b.hmask)
-                    HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice, st),
-                            "tile mask in");
+                if (src[0].raw) {
+                    // A raw file: rows read straight into the pinned buffer,
+                    // the validity derived from the fill on the device.
+                    if ((status = raw_read(*src[0].raw, W, b.hdata, ww, hh, b.stride, rx0, ry0, "src[0]"))) break;
+                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
+                            "tile in");
+                    if (b.dmask)
+                        HIP_TRY(launch_raw_validity(RawTile{b.ddata, b.stride, ww, hh, b.dmask}, src[0].raw->fill, st),
+                                "launching the fill validity kernel");
+                } else {
+                    if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
+                    if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
+                        status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
+                                      (long long)rx0, (long long)ry0);
+                        break;
+                    }
+                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
+                            "tile in");
+                    if (b.hmask)
+                        HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice,
+                                               st),
+                                "tile mask in");
+                }
                 sv.data = b.ddata;
                 sv.width = ww;
                 sv.height = hh;
@@ -992,8 +1072,11 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                 if (dst[o].resident) continue;
                 Buf &b = bufs[nsrc + o];
                 const int64_t cells = (th - 1) * b.stride + tw;
+                if (dst[o].raw && b.dmask)
+                    HIP_TRY(launch_raw_fill(RawTile{b.ddata, b.stride, tw, th, b.dmask}, dst[o].raw->fill, st),
+                            "launching the fill kernel");
                 HIP_TRY(hipMemcpyAsync(b.hdata, b.ddata, (size_t)cells * 4, hipMemcpyDeviceToHost, st), "tile out");
-                if (b.hmask)
+                if (b.hmask && !dst[o].raw)
                     HIP_TRY(hipMemcpyAsync(b.hmask, b.dmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyDeviceToHost, st),
                             "tile mask out");
             }
@@ -1004,7 +1087,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                     waited = true;
                 }
                 Buf &b = bufs[nsrc + o];
-                if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, tr.x0, tr.y0) != 0) {
+                if (dst[o].raw) {
+                    if ((status = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, tr.x0, tr.y0, o))) break;
+                } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, tr.x0, tr.y0) != 0) {
                     status = fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
                                   (long long)tr.x0, (long long)tr.y0);
                     break;
@@ -1168,7 +1253,9 @@ int strata_reduce_chunked(int what, const strata_source *src, const strata_optio
 {
     int rc = check_reduce(what, opts);
     if (rc) return rc;
-    if (!src || (!src->resident && !src->read)) return fail(STRATA_EINVAL, "engine: src[0] is a nil source");
+    if (!src || (!src->resident && !src->raw && !src->read))
+        return fail(STRATA_EINVAL, "engine: src[0] is a nil source");
+    if (!src->resident && src->raw && (rc = check_raw(src->raw, src->masked, "src", 0))) return rc;
     const int64_t W = src->width, H = src->height;
     if (W <= 0 || H <= 0)
         return fail(STRATA_EINVAL, "engine: src[0] is %lld×%lld; sizes must be positive", (long long)W,
@@ -1253,17 +1340,25 @@ int strata_reduce_chunked(int what, const strata_source *src, const strata_optio
             const Rect tr = tile_rect(g, t);
             const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
             tile_counts(tr);
-            if (hmask) memset(hmask, 0, (size_t)words * 8);
-            if (src->read(src->user, hdata, tw, th, stride, hmask, 0, tr.x0, tr.y0) != 0) {
-                status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error", (long long)tr.x0,
-                              (long long)tr.y0);
-                break;
-            }
             const int64_t n = (th - 1) * stride + tw;
-            HIP_TRY(hipMemcpyAsync(ddata, hdata, (size_t)n * 4, hipMemcpyHostToDevice, st), "tile in");
-            if (hmask)
-                HIP_TRY(hipMemcpyAsync(dmask, hmask, (size_t)cdiv(n, 64) * 8, hipMemcpyHostToDevice, st),
-                        "tile mask in");
+            if (src->raw) {
+                if ((status = raw_read(*src->raw, W, hdata, tw, th, stride, tr.x0, tr.y0, "src[0]"))) break;
+                HIP_TRY(hipMemcpyAsync(ddata, hdata, (size_t)n * 4, hipMemcpyHostToDevice, st), "tile in");
+                if (dmask)
+                    HIP_TRY(launch_raw_validity(RawTile{ddata, stride, tw, th, dmask}, src->raw->fill, st),
+                            "launching the fill validity kernel");
+            } else {
+                if (hmask) memset(hmask, 0, (size_t)words * 8);
+                if (src->read(src->user, hdata, tw, th, stride, hmask, 0, tr.x0, tr.y0) != 0) {
+                    status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
+                                  (long long)tr.x0, (long long)tr.y0);
+                    break;
+                }
+                HIP_TRY(hipMemcpyAsync(ddata, hdata, (size_t)n * 4, hipMemcpyHostToDevice, st), "tile in");
+                if (hmask)
+                    HIP_TRY(hipMemcpyAsync(dmask, hmask, (size_t)cdiv(n, 64) * 8, hipMemcpyHostToDevice, st),
+                            "tile mask in");
+            }
             strata_raster tv{};
             tv.data = ddata;
             tv.width = tw;

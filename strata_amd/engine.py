@@ -9,6 +9,7 @@ None on success, a Canceled or SourceError instance otherwise.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
@@ -506,6 +507,214 @@ def _resident(x) -> Optional[Float32Raster]:
     return None
 
 
+# -- raw files (engine/raw.go, engine/rawfile.go) ---------------------------------------
+
+@dataclass
+class RawOptions:
+    """engine.RawOptions (engine/raw.go:16-26): with HasFill, a cell equal to
+    Fill (as a float; a NaN Fill matches any NaN) is NoData."""
+    Fill: float = 0.0
+    HasFill: bool = False
+
+
+class RawFile:
+    """engine.RawFile (engine/rawfile.go:14-40): one file through one or more
+    handles; ReadAt and WriteAt take the handles in turn. The library reads
+    and writes it through the first handle's descriptor."""
+
+    def __init__(self, fds: List[int]):
+        self._fds = list(fds)
+        self._next = 0
+        self._lock = threading.Lock()
+
+    def fileno(self) -> int:
+        return self._fds[0]
+
+    def _fd(self) -> int:
+        with self._lock:
+            fd = self._fds[self._next % len(self._fds)]
+            self._next += 1
+        return fd
+
+    def ReadAt(self, n: int, off: int) -> bytes:
+        return os.pread(self._fd(), n, off)
+
+    def WriteAt(self, b, off: int) -> int:
+        return os.pwrite(self._fd(), b, off)
+
+    def Handles(self) -> int:
+        return len(self._fds)
+
+    def Sync(self) -> None:
+        os.fsync(self._fds[0])
+
+    def Close(self) -> None:
+        for fd in self._fds:
+            os.close(fd)
+        self._fds = []
+
+
+def OpenRawFile(name: str, flag: int, perm: int = 0o644, handles: int = 1) -> RawFile:
+    """engine.OpenRawFile (engine/rawfile.go:42-65): only the first handle
+    creates or truncates; O_APPEND is rejected, WriteAt cannot use it."""
+    if flag & os.O_APPEND:
+        raise OSError(f"open {name}: engine: OpenRawFile: O_APPEND does not allow WriteAt")
+    fds = []
+    try:
+        for i in range(max(1, handles)):
+            if i == 1:
+                flag &= ~(os.O_CREAT | os.O_EXCL | os.O_TRUNC)
+            fds.append(os.open(name, flag, perm))
+    except OSError:
+        for fd in fds:
+            os.close(fd)
+        raise
+    return RawFile(fds)
+
+
+def CreateRawFile(name: str, size: int, perm: int = 0o644, handles: int = 1) -> RawFile:
+    """engine.CreateRawFile (engine/rawfile.go:67-110), without the memory
+    mapping: a new file of size bytes, its blocks allocated at once."""
+    f = OpenRawFile(name, os.O_RDWR | os.O_CREAT | os.O_TRUNC, perm, handles)
+    try:
+        os.ftruncate(f.fileno(), size)
+    except OSError:
+        f.Close()
+        raise
+    return f
+
+
+def _require_raw_size(fn: str, nil: bool, width: int, height: int) -> None:
+    # engine/raw.go:275-285
+    if nil:
+        raise EnginePanic(f"engine: {fn}: nil file")
+    if width <= 0 or height <= 0:
+        raise EnginePanic(f"engine: {fn}: dimensions must be positive, got {width}×{height}")
+    if width > (2 ** 63 - 1) // 4 // height:
+        raise EnginePanic(f"engine: {fn}: {width}×{height} cells do not fit in a file")
+
+
+def _require_region(fn: str, name: str, r: Float32Raster, x: int, y: int, w: int, h: int) -> None:
+    # engine/source.go:193-201
+    err = r.Validate()
+    if err is not None:
+        raise EnginePanic(f"engine: {fn}: {name}: {err}")
+    if x < 0 or y < 0 or x > w - r.Width or y > h - r.Height:
+        raise EnginePanic(f"engine: {fn}: {r.Width}×{r.Height} region at ({x}, {y}) outside {w}×{h} raster")
+
+
+def _read_at(f, n: int, off: int) -> bytes:
+    if hasattr(f, "ReadAt"):
+        return f.ReadAt(n, off)
+    return os.pread(f.fileno(), n, off)
+
+
+def _write_at(f, b, off: int) -> int:
+    if hasattr(f, "WriteAt"):
+        return f.WriteAt(b, off)
+    return os.pwrite(f.fileno(), b, off)
+
+
+def _fill_validity(cells: np.ndarray, fill: float) -> np.ndarray:
+    """vec.ValidBits (internal/vec/validity.go:11-19) for one row."""
+    f = np.float32(fill)
+    if f != f:
+        return ~np.isnan(cells)
+    return cells != f
+
+
+class RawSource(RasterSource):
+    """engine.RawSource (engine/raw.go:28-118): a headerless file of
+    width·height little-endian float32 cells, row-major. A chunked run over
+    a file with a descriptor has the library read it and derive validity
+    from Fill on the device."""
+
+    def __init__(self, f, width: int, height: int, opts: RawOptions = RawOptions()):
+        _require_raw_size("NewRawSource", f is None, width, height)
+        self.f, self.w, self.h, self.opts = f, width, height, opts
+
+    def Size(self):
+        return self.w, self.h
+
+    def Masked(self) -> bool:
+        return self.opts.HasFill
+
+    def ReadWindow(self, ctx, dst: Float32Raster, x: int, y: int):
+        _require_region("RawSource.ReadWindow", "dst", dst, x, y, self.w, self.h)
+        if self.opts.HasFill and dst.Valid is None:
+            raise EnginePanic("engine: RawSource.ReadWindow: the source has a fill value and dst has no validity mask")
+        if ctx is not None and ctx.done():
+            return ctx.err()
+        for row in range(dst.Height):
+            n = 4 * dst.Width
+            b = _read_at(self.f, n, 4 * ((y + row) * self.w + x))
+            if len(b) < n:
+                return SourceError(f"engine: raw source: reading row {y + row}: unexpected EOF")
+            cells = np.frombuffer(b, dtype="<f4").astype(np.float32)
+            dst.Data[row * dst.Stride:row * dst.Stride + dst.Width] = cells
+            if dst.Valid is not None:
+                bits = _fill_validity(cells, self.opts.Fill) if self.opts.HasFill else np.ones(dst.Width, bool)
+                _put_bits(dst.Valid, dst.ValidOffset + row * dst.Stride, bits)
+        return None
+
+
+class RawSink(RasterSink):
+    """engine.RawSink (engine/raw.go:164-245): writes rows into the file;
+    with HasFill, Fill goes into every invalid cell first (in the source
+    raster itself, as fillInvalid does)."""
+
+    def __init__(self, f, width: int, height: int, opts: RawOptions = RawOptions()):
+        _require_raw_size("NewRawSink", f is None, width, height)
+        self.f, self.w, self.h, self.opts = f, width, height, opts
+
+    def Size(self):
+        return self.w, self.h
+
+    def Masked(self) -> bool:
+        return self.opts.HasFill
+
+    def WriteWindow(self, ctx, src: Float32Raster, x: int, y: int):
+        _require_region("RawSink.WriteWindow", "src", src, x, y, self.w, self.h)
+        if src.Valid is not None and not self.opts.HasFill:
+            raise EnginePanic("engine: RawSink.WriteWindow: src has a validity mask and the sink has no fill value")
+        if ctx is not None and ctx.done():
+            return ctx.err()
+        from .raster import unpack_bits
+        for row in range(src.Height):
+            s0 = row * src.Stride
+            cells = src.Data[s0:s0 + src.Width]
+            if src.Valid is not None:
+                bad = ~unpack_bits(src.Valid, src.ValidOffset + s0, src.Width)
+                cells[bad] = np.float32(self.opts.Fill)
+            b = np.ascontiguousarray(cells, dtype="<f4").tobytes()
+            n = _write_at(self.f, b, 4 * ((y + row) * self.w + x))
+            if n < len(b):
+                return SourceError(f"engine: raw sink: writing row {y + row}: short write")
+        return None
+
+
+def NewRawSource(f, width: int, height: int, opts: RawOptions = RawOptions()) -> RawSource:
+    return RawSource(f, width, height, opts)
+
+
+def NewRawSink(f, width: int, height: int, opts: RawOptions = RawOptions()) -> RawSink:
+    return RawSink(f, width, height, opts)
+
+
+def _raw(x) -> Optional[_lib.Raw]:
+    """The library's descriptor of a raw source or sink over a file with a
+    descriptor, or None."""
+    if not isinstance(x, (RawSource, RawSink)) or not hasattr(x.f, "fileno"):
+        return None
+    try:
+        fd = x.f.fileno()
+    except (OSError, ValueError):
+        return None
+    r = _lib.Raw()
+    r.fd, r.has_fill, r.fill, r.offset = fd, 1 if x.opts.HasFill else 0, float(np.float32(x.opts.Fill)), 0
+    return r
+
+
 def _read_callback(ctx, s: RasterSource, j: int, errors: List[Exception]):
     """The read callback of source j: ReadWindow into the library's pinned
     buffer; an error or a panic is kept in errors and reported as a
@@ -605,10 +814,14 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
         c = csrc[j]
         c.width, c.height, c.masked = sw, sh, 1 if s.Masked() else 0
         res = _resident(s)
+        raw = _raw(s)
         if res is not None:
             rc_ = res.to_c()
             keep.append(rc_)
             c.resident = ctypes.pointer(rc_)
+        elif raw is not None:
+            keep.append(raw)
+            c.raw = ctypes.pointer(raw)
         else:
             fn = make_read(s, j)
             keep.append(fn)
@@ -619,10 +832,14 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
         c = cdst[j]
         c.width, c.height, c.masked = sw, sh, 1 if d.Masked() else 0
         res = _resident(d)
+        raw = _raw(d)
         if res is not None:
             rc_ = res.to_c()
             keep.append(rc_)
             c.resident = ctypes.pointer(rc_)
+        elif raw is not None:
+            keep.append(raw)
+            c.raw = ctypes.pointer(raw)
         else:
             fn = make_write(d, j)
             keep.append(fn)
@@ -715,10 +932,14 @@ def ReduceChunked(ctx: Optional[Context], src: List[RasterSource], what: int,
     c = _lib.Source()
     c.width, c.height, c.masked = w, h, 1 if s.Masked() else 0
     res = _resident(s)
+    raw = _raw(s)
     if res is not None:
         rc_ = res.to_c()
         keep.append(rc_)
         c.resident = ctypes.pointer(rc_)
+    elif raw is not None:
+        keep.append(raw)
+        c.raw = ctypes.pointer(raw)
     else:
         fn = _read_callback(ctx, s, 0, errors)
         keep.append(fn)
