@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--focal-steps", type=int, default=5, help="timed runs of each focal op")
     ap.add_argument("--reduce-steps", type=int, default=10,
                     help="timed runs of each reduction over the DEM (0: skip)")
+    ap.add_argument("--raw-size", type=int, default=8192,
+                    help="side of the file-to-file SlopeChunked run (raw float32 files; 0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -225,6 +227,10 @@ def main() -> int:
     if args.reduce_steps > 0:
         reduce_res = reduce_bench(args, dem, N, log)
 
+    raw_res = None
+    if args.raw_size > 0:
+        raw_res = raw_bench(args, dev, log)
+
     batch = None
     if args.batch_tiles > 0:
         del out, sink
@@ -279,6 +285,7 @@ def main() -> int:
         "serving": serving,
         "focal": focal_res,
         "reduce": reduce_res,
+        "file_to_file": raw_res,
     }
     if serving and serving["levels"]:
         at = min(serving["levels"], key=lambda lv: abs(lv["qps"] - 1000.0))
@@ -287,6 +294,53 @@ def main() -> int:
                                    "arrival to completion"}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def raw_bench(args, dev, log):
+    """File-to-file SlopeChunked: an fBm DEM written as a raw float32 file
+    with a fill value (0.5% blob NoData), read tile by tile by the library,
+    its NoData derived on the device, the slope written to a second raw file
+    with the fill in its invalid cells. Wall time of the whole call (file
+    reads and writes in the page cache, PCIe both ways, kernels)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from strata_amd import synth, terrain
+    from strata_amd.engine import CreateRawFile, OpenRawFile, Options, RawOptions, RawSink, RawSource
+
+    N = args.raw_size
+    fill = -9999.0
+    data = synth.fbm_dem(N, N, hurst=0.8, seed=1, lo=0.0, hi=3000.0, device=dev)
+    valid, _ = synth.blob_nodata(N, N, frac=0.005, rmin=5, rmax=50, seed=2, device=dev)
+    idx = torch.arange(N * N, device=dev)
+    bits = ((valid[idx >> 6] >> (idx & 63)) & 1).bool()
+    data = torch.where(bits, data, torch.tensor(fill, device=dev))
+    opts = RawOptions(Fill=fill, HasFill=True)
+    res = {}
+    with tempfile.TemporaryDirectory(prefix="strata_raw_") as d:
+        ipath, opath = os.path.join(d, "dem.raw"), os.path.join(d, "slope.raw")
+        data.cpu().numpy().astype("<f4").tofile(ipath)
+        del data, bits, idx
+        fi = OpenRawFile(ipath, os.O_RDONLY)
+        fo = CreateRawFile(opath, 4 * N * N)
+        k = terrain.SlopeOptions(CellSize=30)
+        runs = []
+        for rep in range(3):
+            t = time.perf_counter()
+            err = terrain.SlopeChunked(None, RawSink(fo, N, N, opts), RawSource(fi, N, N, opts), k,
+                                       Options(TileHeight=args.tile_height))
+            runs.append(time.perf_counter() - t)
+            assert err is None, err
+        fi.Close()
+        fo.Close()
+        el = min(runs)
+        res = {"size": N, "tile_height": args.tile_height, "s": round(el, 4),
+               "gcells_s": round(N * N / el / 1e9, 3), "file_gb_s": round(8 * N * N / el / 1e9, 3),
+               "runs_s": [round(r, 4) for r in runs]}
+    log(f"bench: file-to-file {res}")
+    return res
 
 
 def reduce_bench(args, dem, N, log):

@@ -927,7 +927,14 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         float *ddata = nullptr;
         uint64_t *dmask = nullptr;
     };
-    std::vector<Buf> bufs(nsrc + ndst);
+    // Two sets of buffers: while the device computes one tile, the host
+    // hands the previous tile's outputs to their sinks and reads the next
+    // tile (the write-behind of internal/exec/behind.go:54-192).
+    constexpr int kSlots = 2;
+    const int nbuf = nsrc + ndst;
+    std::vector<Buf> bufs(kSlots * nbuf);
+    hipEvent_t done[kSlots] = {nullptr, nullptr};
+    auto buf = [&](int slot, int i) -> Buf & { return bufs[slot * nbuf + i]; };
     auto free_bufs = [&]() {
         for (auto &b : bufs) {
             if (b.hdata) (void)hipHostFree(b.hdata);
@@ -935,6 +942,8 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             if (b.ddata) (void)hipFree(b.ddata);
             if (b.dmask) (void)hipFree(b.dmask);
         }
+        for (auto &e : done)
+            if (e) (void)hipEventDestroy(e);
     };
     struct Guard {
         std::function<void()> f;
@@ -955,11 +964,6 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     int status = STRATA_OK;
     {
         Guard guard{free_bufs};
-        for (int i = 0; i < nsrc; ++i)
-            if (!src[i].resident && (rc = alloc_buf(bufs[i], inW, inH, src[i].masked))) return rc;
-        for (int o = 0; o < ndst; ++o)
-            if (!dst[o].resident && (rc = alloc_buf(bufs[nsrc + o], g.tileW, g.tileH, dst[o].masked))) return rc;
-
         strata_stats s{};
         bool all_resident = true;
         for (int i = 0; i < nsrc; ++i) all_resident = all_resident && src[i].resident;
@@ -987,27 +991,66 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             g_err.clear();
             return STRATA_OK;
         }
+        for (int slot = 0; slot < kSlots; ++slot) {
+            for (int i = 0; i < nsrc; ++i)
+                if (!src[i].resident && (rc = alloc_buf(buf(slot, i), inW, inH, src[i].masked))) return rc;
+            for (int o = 0; o < ndst; ++o)
+                if (!dst[o].resident && (rc = alloc_buf(buf(slot, nsrc + o), g.tileW, g.tileH, dst[o].masked)))
+                    return rc;
+            HIP_TRY(hipEventCreateWithFlags(&done[slot], hipEventDisableTiming), "tile event");
+        }
+
+        // The tile whose outputs wait to be handed to the sinks.
+        struct Pending {
+            bool live = false;
+            int slot = 0;
+            Rect tr{};
+        } pend;
+        auto finish = [&](Pending &p) -> int {
+            if (!p.live) return STRATA_OK;
+            p.live = false;
+            HIP_TRY(hipEventSynchronize(done[p.slot]), "waiting for a tile");
+            const int64_t tw = p.tr.x1 - p.tr.x0, th = p.tr.y1 - p.tr.y0;
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident) continue;
+                Buf &b = buf(p.slot, nsrc + o);
+                if (dst[o].raw) {
+                    int e = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, p.tr.x0, p.tr.y0, o);
+                    if (e) return e;
+                } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, p.tr.x0, p.tr.y0) != 0) {
+                    return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
+                                (long long)p.tr.x0, (long long)p.tr.y0);
+                }
+            }
+            tile_stats(s, r, rmin, g, nsrc, ndst, p.tr);
+            return STRATA_OK;
+        };
+
         for (int64_t t = 0; t < g.tiles; ++t) {
             if (cancelled(opts)) {
                 status = STRATA_CANCELLED;
                 break;
             }
+            const int slot = (int)(t % kSlots);
             const Rect tr = tile_rect(g, t);
             const int64_t rx0 = std::max<int64_t>(0, tr.x0 - r), ry0 = std::max<int64_t>(0, tr.y0 - r);
             const int64_t rx1 = std::min<int64_t>(W, tr.x1 + r), ry1 = std::min<int64_t>(H, tr.y1 + r);
             const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
 
-            // The source window: resident (read in place) or read by the
-            // callback into the pinned buffer and copied to the device.
+            // The source window: resident (read in place) or read into the
+            // slot's pinned buffer and copied to the device. The slot's last
+            // tile was handed off by the previous iteration's finish().
             strata_raster sv{};
             int64_t sx = 0, sy = 0;
             if (src[0].resident) {
                 sv = *src[0].resident;
-                if (sv.mem == STRATA_MEM_HOST)
-                    return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+                if (sv.mem == STRATA_MEM_HOST) {
+                    status = fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+                    break;
+                }
                 if (!src[0].masked) sv.valid = nullptr;
             } else {
-                Buf &b = bufs[0];
+                Buf &b = buf(slot, 0);
                 const int64_t ww = rx1 - rx0, hh = ry1 - ry0;
                 const int64_t cells = (hh - 1) * b.stride + ww;
                 if (src[0].raw) {
@@ -1045,15 +1088,14 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             strata_raster dv[kMaxDst]{};
             Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
             for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
-            bool buffered = !src[0].resident;
+            bool bad_sink = false;
             for (int o = 0; o < ndst; ++o) {
                 if (dst[o].resident) {
                     dv[o] = *dst[o].resident;
-                    if (dv[o].mem == STRATA_MEM_HOST)
-                        return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                    if (dv[o].mem == STRATA_MEM_HOST) bad_sink = true;
                     if (!dst[o].masked) dv[o].valid = nullptr;
                 } else {
-                    Buf &b = bufs[nsrc + o];
+                    Buf &b = buf(slot, nsrc + o);
                     dv[o].data = b.ddata;
                     dv[o].width = tw;
                     dv[o].height = th;
@@ -1062,15 +1104,17 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                     dv[o].valid_offset = 0;
                     op.dx[o] = tr.x0;
                     op.dy[o] = tr.y0;
-                    buffered = true;
                 }
             }
+            if (bad_sink) {
+                status = fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                break;
+            }
             if ((rc = run_rect(J, op, tr, st))) return rc;
-            // Hand buffered outputs to their sinks.
-            bool waited = false;
+            // Buffered outputs back to the host: a raw sink's fill first.
             for (int o = 0; o < ndst; ++o) {
                 if (dst[o].resident) continue;
-                Buf &b = bufs[nsrc + o];
+                Buf &b = buf(slot, nsrc + o);
                 const int64_t cells = (th - 1) * b.stride + tw;
                 if (dst[o].raw && b.dmask)
                     HIP_TRY(launch_raw_fill(RawTile{b.ddata, b.stride, tw, th, b.dmask}, dst[o].raw->fill, st),
@@ -1080,27 +1124,18 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                     HIP_TRY(hipMemcpyAsync(b.hmask, b.dmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyDeviceToHost, st),
                             "tile mask out");
             }
-            for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident) continue;
-                if (!waited) {
-                    HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
-                    waited = true;
-                }
-                Buf &b = bufs[nsrc + o];
-                if (dst[o].raw) {
-                    if ((status = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, tr.x0, tr.y0, o))) break;
-                } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, tr.x0, tr.y0) != 0) {
-                    status = fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
-                                  (long long)tr.x0, (long long)tr.y0);
-                    break;
-                }
-            }
-            if (status != STRATA_OK) break;
-            // A callback source's pinned buffer is reused by the next tile,
-            // and a cancellable call observes its flag per finished tile.
-            if (!waited && (buffered || opts_cancel)) HIP_TRY(hipStreamSynchronize(st), "waiting for a tile");
-            tile_stats(s, r, rmin, g, nsrc, ndst, tr);
+            HIP_TRY(hipEventRecord(done[slot], st), "tile event");
+            // While the device runs this tile, the previous one goes out.
+            if ((status = finish(pend))) break;
+            pend.live = true;
+            pend.slot = slot;
+            pend.tr = tr;
+            // A cancellable call observes its flag once per finished tile.
+            if (opts_cancel && (status = finish(pend))) break;
         }
+        // The claimed tile always completes (internal/exec/worker.go:112-181).
+        const int last = finish(pend);
+        if (status == STRATA_OK || status == STRATA_CANCELLED) status = last ? last : status;
         HIP_TRY(hipStreamSynchronize(st), "synchronizing");
         add_stats(stats, s, nsrc, ndst);
     }
