@@ -24,7 +24,7 @@ struct strata_batch {
     std::vector<std::vector<float>> taps;     // copies of every kernel's taps
     std::vector<BatchOp> ops;                 // distinct ops
     std::vector<int32_t> op_of;               // op index of each task
-    std::vector<int32_t> order;               // tasks in work-list order (largest first)
+    std::vector<int32_t> order;               // tasks in work-list order (by source address)
     std::vector<BatchTask> list;              // tasks in work-list order (made by make_task)
     std::vector<char> image;                  // host image of the device work lists
     ListLayout layout{};
@@ -350,11 +350,16 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
         b->op_of[i] = found;
         b->cells += b->tasks[i].src.width * b->tasks[i].src.height;
     }
-    // Largest tasks first, so the tail of the work list is small units.
+    // Tasks in the order of their source's address: windows of one raster
+    // in row-major order of their corners. The waves running at any moment
+    // then read nearby rows of the parent, a few pages of it, instead of
+    // windows scattered over the whole raster (which thrashes the address
+    // translation caches). Units are at most 128x128 cells, so the tail of
+    // the list is short whatever the order.
     b->order.resize(ntasks);
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
-        return b->tasks[a].src.width * b->tasks[a].src.height > b->tasks[c].src.width * b->tasks[c].src.height;
+        return (uintptr_t)b->tasks[a].src.data < (uintptr_t)b->tasks[c].src.data;
     });
     b->list.resize(ntasks);
     for (int n = 0; n < ntasks; ++n) b->list[n] = make_task(b->tasks[b->order[n]], b->op_of[b->order[n]]);

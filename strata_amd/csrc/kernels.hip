@@ -1370,28 +1370,34 @@ __global__ __launch_bounds__(64) void batch_clear(unsigned *counters)
     if (threadIdx.x < 2) counters[threadIdx.x * kCounterStride] = 0u;
 }
 
-// Sweep units, then validity units, claimed one wave at a time from one
-// counter: lane 0 claims, readfirstlane broadcasts, and a wave never waits
-// for the rest of its workgroup. The lists and ops are read-only for the
-// whole launch and never alias the rasters the units write (__restrict__),
-// so their fields stay in registers across the units' stores instead of
-// being reloaded after each one.
+// Sweep units, then validity units, claimed four at a time by a
+// workgroup: its four waves take four consecutive units, which for a
+// radius-1 task are 512 adjacent columns of the same rows (the shape of
+// horn_sweep's workgroups, so rows stream as whole DRAM pages and the
+// waves share their boundary lines in L1). The next claim is made before
+// the current units run, into the other of two LDS slots. The lists and
+// ops are read-only for the whole launch and never alias the rasters the
+// units write (__restrict__), so their fields stay in registers across
+// the units' stores instead of being reloaded after each one.
 __global__ __launch_bounds__(256) void batch_wave_kernel(const BatchHeader *__restrict__ hdr,
                                                          const BatchTask *__restrict__ sweeps,
                                                          const BatchTask *__restrict__ valids,
                                                          const BatchOp *__restrict__ ops, unsigned *counter)
 {
+    __shared__ unsigned s_claim[2];
     const int64_t nh = hdr[BF_HORN].nunits, nv = hdr[BF_VALIDITY].nunits;
     const int64_t th = hdr[BF_HORN].ntasks, tv = hdr[BF_VALIDITY].ntasks;
-    const bool lead = (threadIdx.x & 63) == 0;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int64_t ti = 0, tu0 = 0, tu1 = 0;  // the current task and its units [tu0, tu1) in its list
     bool in_valid = false;            // which list the cached task belongs to
-    unsigned c = 0u;
-    if (lead) c = atomicAdd(counter, 1u);
-    int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(c);
-    while (u < nh + nv) {
-        unsigned next = 0u;
-        if (lead) next = atomicAdd(counter, 1u);  // consumed after this unit
+    if (threadIdx.x == 0) s_claim[0] = atomicAdd(counter, 4u);
+    __syncthreads();
+    int slot = 0;
+    int64_t base = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(s_claim[0]);
+    while (base < nh + nv) {
+        if (threadIdx.x == 0) s_claim[slot ^ 1] = atomicAdd(counter, 4u);  // consumed after these units
+        const int64_t u = base + wv;
+        if (u < nh + nv) {
         const bool valid = u >= nh;
         const BatchTask *list = valid ? valids : sweeps;
         const int64_t lu_all = valid ? u - nh : u;
@@ -1414,7 +1420,10 @@ __global__ __launch_bounds__(256) void batch_wave_kernel(const BatchHeader *__re
         } else {
             batch_validity(op, t, lu);
         }
-        u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(next);
+        }
+        __syncthreads();  // the next claim is visible, and every wave is done with these units
+        slot ^= 1;
+        base = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(s_claim[slot]);
     }
 }
 
