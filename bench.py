@@ -226,6 +226,9 @@ def main() -> int:
     reduce_res = None
     if args.reduce_steps > 0:
         reduce_res = reduce_bench(args, dem, N, log)
+        del out, sink
+        torch.cuda.empty_cache()
+        reduce_res["folded_stats"] = folded_bench(args, dem, N, dev, log)
 
     raw_res = None
     if args.raw_size > 0:
@@ -233,7 +236,7 @@ def main() -> int:
 
     batch = None
     if args.batch_tiles > 0:
-        del out, sink
+        out = sink = None
         torch.cuda.empty_cache()
         batch = batch_bench(args, dem, N, dev, log)
 
@@ -372,6 +375,75 @@ def reduce_bench(args, dem, N, log):
     s = rd.Stats(dem)
     res["summary"] = {"count": s.Count, "sum": s.Sum, "mean": s.Mean, "stddev": s.StdDev, "min": float(s.Min),
                       "max": float(s.Max)}
+    return res
+
+
+def folded_bench(args, dem, N, dev, log):
+    """Slope's Stats folded into the Slope pass through a tee sink
+    (graph/sinks.go): alone (no raster written), and with the raster
+    written too, against the op followed by reduce.Stats of its output.
+    Whole-call wall times; the resident pass's step of rows is swept for
+    the alone form."""
+    import torch
+
+    from strata_amd import _lib, terrain
+    from strata_amd import reduce as rd
+    from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked, TeeSink
+    from strata_amd.raster import Float32Raster
+
+    cells = N * N
+    slope = terrain.slopeOp(terrain.SlopeOptions(CellSize=30))
+    out = Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                        torch.zeros((cells + 63) // 64 + 1, dtype=torch.int64, device=dev), 0)
+    opts = Options(TileHeight=args.tile_height)
+
+    def alone():
+        tee = TeeSink(N, N, True)
+        assert ProcessChunked(None, [tee], [MemorySource(dem)], slope, opts) is None
+        return tee
+
+    def written():
+        tee = TeeSink(N, N, True, [MemorySink(out)])
+        assert ProcessChunked(None, [tee], [MemorySource(dem)], slope, opts) is None
+        return tee
+
+    def separate():
+        assert ProcessChunked(None, [MemorySink(out)], [MemorySource(dem)], slope, opts) is None
+        return rd.Stats(out)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.reduce_steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.reduce_steps
+
+    L = _lib.lib()
+    saved = _lib.Tuning()
+    L.strata_get_tuning(saved)
+    sweep = {}
+    for mb in (64, 256, 1024, 4096):
+        t = _lib.Tuning()
+        L.strata_get_tuning(t)
+        t.fold_stripe_mb = mb
+        assert L.strata_set_tuning(t) == 0, _lib.last_error()
+        sweep[mb] = round(timed(alone) * 1e3, 4)
+    L.strata_set_tuning(saved)
+    res = {"step_mb_sweep_ms": sweep}
+    for name, fn in (("alone", alone), ("written", written), ("separate", separate)):
+        el = timed(fn)
+        b = (4.125 if name == "alone" else 8.25) * cells
+        res[name] = {"ms": round(el * 1e3, 4), "gcells_s": round(cells / el / 1e9, 2),
+                     "hbm_frac": round(b / el / 1e9 / HBM_PEAK_GBS, 4)}
+    want = separate()
+    res["identical"] = rd.Folded(alone()) == want and rd.Folded(written()) == want
+    res["speedup_alone"] = round(res["separate"]["ms"] / res["alone"]["ms"], 3)
+    log(f"bench: folded stats {res}")
+    del out
+    torch.cuda.empty_cache()
     return res
 
 

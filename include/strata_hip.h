@@ -170,6 +170,11 @@ const char *strata_last_error(void);
 /* Version string of the library build. */
 const char *strata_version(void);
 
+/* Returns the scratch that chunked calls keep for reuse (the device
+ * buffers of fold-only sinks) to the device. No counterpart in the
+ * reference. Not to be called while a call runs. */
+int strata_trim(void);
+
 /*
  * exec.ProcessN (internal/exec/process.go:28): run k over every cell of
  * dst[0..ndst) from src[0..nsrc). Same checks (process.go:37-135, minus
@@ -228,14 +233,26 @@ typedef struct strata_source {
     const strata_raw *raw;   /* used when resident is NULL               */
 } strata_source;
 
+/* A sink may also fold what it is given, as graph's teeSink does
+ * (graph/sinks.go:17-71): with fold != NULL, every tile of the output is
+ * folded on the device with the reduction fold_what (STRATA_REDUCE_*,
+ * below) before the sink receives it, and the combined partial is stored
+ * into *fold when the call completes (zeroed otherwise). Only the tile's
+ * valid cells take part when the sink is masked, every cell (the NaN ring
+ * included) when it is not: exec.Reduce over the window the sink is
+ * handed. A sink with a fold and no resident raster, raw file or write
+ * callback receives nothing else: statistics alone write no raster. */
+struct strata_fold;
+
 typedef struct strata_sink {
     int64_t width, height;
     int32_t masked;
-    int32_t reserved;
+    int32_t fold_what;       /* STRATA_REDUCE_* bits when fold != NULL   */
     strata_write_fn write;   /* used when resident and raw are NULL      */
     void *user;
     const strata_raster *resident;
     const strata_raw *raw;   /* used when resident is NULL               */
+    struct strata_fold *fold;
 } strata_sink;
 
 /*
@@ -341,7 +358,13 @@ int strata_reduce_chunked(int what, const strata_source *src, const strata_optio
 /* ------------------------------------------------------------------ */
 typedef struct strata_tuning {
     int32_t sweep_rows;  /* rows per block of the radius-1 terrain kernels (4..4096) */
-    int32_t reserved[7];
+    /* MiB of output per pass step when a chunked call over device-resident
+     * operands folds an output it writes nowhere (a fold-only sink): the
+     * rows go through a device buffer of about this size, kept for reuse
+     * (strata_trim). Smaller steps hold less memory but launch more,
+     * shorter passes (1..4096, default 4096). */
+    int32_t fold_stripe_mb;
+    int32_t reserved[6];
 } strata_tuning;
 
 int strata_get_tuning(strata_tuning *t);

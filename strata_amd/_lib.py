@@ -103,7 +103,7 @@ class Stats(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("sweep_rows", c_int32), ("reserved", c_int32 * 7)]
+    _fields_ = [("sweep_rows", c_int32), ("fold_stripe_mb", c_int32), ("reserved", c_int32 * 6)]
 
 
 READ_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
@@ -137,16 +137,7 @@ class Source(ctypes.Structure):
 
 
 class Sink(ctypes.Structure):
-    _fields_ = [
-        ("width", c_int64),
-        ("height", c_int64),
-        ("masked", c_int32),
-        ("reserved", c_int32),
-        ("write", WRITE_FN),
-        ("user", c_void_p),
-        ("resident", POINTER(Raster)),
-        ("raw", POINTER(Raw)),
-    ]
+    """strata_sink; its fields are set below, after Fold."""
 
 
 class Task(ctypes.Structure):
@@ -239,6 +230,19 @@ class Fold(ctypes.Structure):
     ]
 
 
+Sink._fields_ = [
+    ("width", c_int64),
+    ("height", c_int64),
+    ("masked", c_int32),
+    ("fold_what", c_int32),
+    ("write", WRITE_FN),
+    ("user", c_void_p),
+    ("resident", POINTER(Raster)),
+    ("raw", POINTER(Raw)),
+    ("fold", POINTER(Fold)),
+]
+
+
 def limbs_to_int(limbs) -> int:
     """The signed integer of two's complement 64-bit limbs."""
     v = 0
@@ -253,6 +257,7 @@ EXPORTS = (
     "strata_init",
     "strata_last_error",
     "strata_version",
+    "strata_trim",
     "strata_process",
     "strata_process_chunked",
     "strata_plan_stats",
@@ -301,6 +306,8 @@ def lib() -> ctypes.CDLL:
     L.strata_init.restype = c_int
     L.strata_last_error.argtypes = []
     L.strata_last_error.restype = c_char_p
+    L.strata_trim.argtypes = []
+    L.strata_trim.restype = c_int
     L.strata_version.argtypes = []
     L.strata_version.restype = c_char_p
     L.strata_process.argtypes = [POINTER(Kernel), POINTER(Raster), c_int, POINTER(Raster), c_int,

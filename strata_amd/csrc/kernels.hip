@@ -1470,7 +1470,7 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
 
 // Launch-shape tuning (never changes a result).
-static Tuning g_tuning{32, {0, 0, 0, 0, 0, 0, 0}};
+static Tuning g_tuning{32, 4096, {0, 0, 0, 0, 0, 0}};
 static int sweep_rows() { return g_tuning.sweep_rows; }
 
 Tuning get_tuning() { return g_tuning; }
@@ -1478,6 +1478,7 @@ Tuning get_tuning() { return g_tuning; }
 bool set_tuning(const Tuning &t)
 {
     if (t.sweep_rows < 4 || t.sweep_rows > 4096) return false;
+    if (t.fold_stripe_mb < 1 || t.fold_stripe_mb > 4096) return false;
     g_tuning = t;
     return true;
 }

@@ -73,8 +73,9 @@ struct Params {
 
 // Launch-shape knobs (results never depend on them).
 struct Tuning {
-    int32_t sweep_rows;  // rows per block of the radius-1 sweep
-    int32_t reserved[7];
+    int32_t sweep_rows;      // rows per block of the radius-1 sweep
+    int32_t fold_stripe_mb;  // output MiB per step of a resident fold-only pass
+    int32_t reserved[6];
 };
 Tuning get_tuning();
 bool set_tuning(const Tuning &t);

@@ -60,6 +60,19 @@ __device__ __forceinline__ int wave_sum(int x)
     return __builtin_amdgcn_readlane(x, 63);
 }
 
+// Maximum over the wave of x, returned to every lane: wave_sum's pattern
+// with max (0, what disabled lanes read, is its identity).
+__device__ __forceinline__ uint32_t wave_max(uint32_t x)
+{
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xe, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xc, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, true));
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // True on every lane of the wave (a ballot; p is 0 or 1).
 __device__ __forceinline__ bool wave_all(uint32_t p) { return __builtin_amdgcn_ballot_w64(p == 0u) == 0; }
 
@@ -268,20 +281,23 @@ __global__ __launch_bounds__(kFoldThreads) void fold_kernel(FoldArgs a, int64_t 
 #pragma unroll
             for (int j = 0; j < kStepLoads; ++j) key[j] = ok[j] ? tally(L, v[j]) : 0u;
             if constexpr (BIN) {
-                // A window of 8 exponent bins [base, base + 8) from the first
-                // lane's first key. When every key of the step lies in it, a
-                // lane adds sig·2^(e - base) (and the square halves at
-                // 2^(2(e - base))) in 64 bits and the wave sums those once.
+                // A window of kWin exponent bins (base, the wave's largest
+                // key] below the step's largest key. When every key of the
+                // step lies in it, a lane adds sig·2^(e - base) (and the
+                // square halves at 2^(2(e - base))) in 64 bits and the wave
+                // sums those once. A lane's sums stay below 2^47: 8 loads of
+                // 2^24·2^19 (sums alone, kWin 20), or of 2^24·2^20 (square
+                // halves, kWin 11), so each wave_sum64 half stays below 2^31.
+                constexpr uint32_t kWin = SQ ? 11u : 20u;
                 uint32_t k1 = 0;
 #pragma unroll
-                for (int j = kStepLoads - 1; j >= 0; --j) k1 = key[j] ? key[j] : k1;
-                const uint64_t any = __builtin_amdgcn_ballot_w64(k1 != 0u);
-                if (any != 0) {
-                    const uint32_t k0 = __builtin_amdgcn_readlane(k1, __builtin_ctzll(any));
-                    const uint32_t base = (k0 > 4u ? k0 : 5u) - 4u;
+                for (int j = 0; j < kStepLoads; ++j) k1 = max(k1, key[j]);
+                const uint32_t kmax = wave_max(k1);
+                if (kmax != 0u) {
+                    const uint32_t base = (kmax > kWin ? kmax : kWin) - (kWin - 1u);
                     uint32_t inwin = 1u;
 #pragma unroll
-                    for (int j = 0; j < kStepLoads; ++j) inwin &= (key[j] == 0u || key[j] - base < 8u) ? 1u : 0u;
+                    for (int j = 0; j < kStepLoads; ++j) inwin &= (key[j] == 0u || key[j] >= base) ? 1u : 0u;
                     if (wave_all(inwin)) {
                         long long S = 0, Lq = 0, Hq = 0;
 #pragma unroll

@@ -315,7 +315,7 @@ int stage_out(const strata_raster &h, Staged &s, hipStream_t st)
     return STRATA_OK;
 }
 
-SrcView src_view(const strata_raster &r, int64_t sx, int64_t sy, int64_t W, int64_t H)
+SrcView srcview(const strata_raster &r, int64_t sx, int64_t sy, int64_t W, int64_t H)
 {
     SrcView v;
     v.p = r.data;
@@ -510,7 +510,7 @@ int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
 {
     Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
     if (rg.w <= 0 || rg.h <= 0) return STRATA_OK;
-    const SrcView s = src_view(*op.src, op.sx, op.sy, op.W, op.H);
+    const SrcView s = srcview(*op.src, op.sx, op.sy, op.W, op.H);
     const bool timed = g_timing.on;
     for (const Group &g : J.groups) {
         DstView d[kMaxOutputs];
@@ -587,6 +587,40 @@ int ensure_device()
     return STRATA_OK;
 }
 
+// Stream-ordered scratch of the chunked passes (the buffers of outputs
+// that are only folded) comes from a pool of the library's own per device
+// that keeps what calls free: a later call of the same shape reuses the
+// memory instead of mapping it again. The pool holds at most the largest
+// scratch one call used, until strata_trim.
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pools[kMaxDevices] = {};
+
+int scratch_pool(hipMemPool_t *out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(STRATA_EDEVICE, "strata: device %d past the pool table", dev);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pools[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        HIP_TRY(hipMemPoolCreate(&pool, &props), "creating the scratch pool");
+        uint64_t keep = UINT64_MAX;
+        const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return hip_fail(e, "configuring the scratch pool");
+        }
+        g_pools[dev] = pool;
+    }
+    *out = g_pools[dev];
+    return STRATA_OK;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -606,6 +640,15 @@ int strata_init(int device)
 const char *strata_last_error(void) { return g_err.c_str(); }
 
 const char *strata_version(void) { return "strata-hip 0.1 gfx950"; }
+
+int strata_trim(void)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (hipMemPool_t p : g_pools)
+        if (p) HIP_TRY(hipMemPoolTrimTo(p, 0), "trimming the scratch pool");
+    g_err.clear();
+    return STRATA_OK;
+}
 
 int strata_selfcheck(int kind, uint64_t lo, uint64_t n, uint64_t *mismatches, uint64_t *first, uint64_t *checked)
 {
@@ -640,7 +683,7 @@ int strata_correlate_error(const strata_kernel *k, const strata_raster *src, con
                     (long long)src->height, (long long)out->width, (long long)out->height);
     const Params P = make_params(k);
     const Region rg{src->width, src->height, 0, 0, src->width, src->height};
-    const SrcView sv = src_view(*src, 0, 0, src->width, src->height);
+    const SrcView sv = srcview(*src, 0, 0, src->width, src->height);
     unsigned long long *d = nullptr;
     HIP_TRY(hipMalloc((void **)&d, 4 * sizeof(unsigned long long)), "error report buffer");
     unsigned long long h[4] = {0, 0, 0, 0};
@@ -665,6 +708,7 @@ int strata_get_tuning(strata_tuning *t)
     const Tuning g = get_tuning();
     memset(t, 0, sizeof *t);
     t->sweep_rows = g.sweep_rows;
+    t->fold_stripe_mb = g.fold_stripe_mb;
     return STRATA_OK;
 }
 
@@ -673,7 +717,10 @@ int strata_set_tuning(const strata_tuning *t)
     if (!t) return fail(STRATA_EINVAL, "strata: nil tuning");
     Tuning g = get_tuning();
     g.sweep_rows = t->sweep_rows;
-    if (!set_tuning(g)) return fail(STRATA_EINVAL, "strata: sweep_rows %d outside [4, 4096]", t->sweep_rows);
+    g.fold_stripe_mb = t->fold_stripe_mb;
+    if (!set_tuning(g))
+        return fail(STRATA_EINVAL, "strata: sweep_rows %d outside [4, 4096] or fold_stripe_mb %d outside [1, 4096]",
+                    t->sweep_rows, t->fold_stripe_mb);
     g_err.clear();
     return STRATA_OK;
 }
@@ -872,277 +919,6 @@ int process(const Job &J, const strata_raster *dst, int ndst, const strata_raste
     return status;
 }
 
-int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata_source *src, int nsrc,
-                    const strata_options *opts, strata_stats *stats)
-{
-    int rc = STRATA_OK;
-    if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
-    if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
-        return fail(STRATA_EINVAL, "engine: negative Options");
-    const int64_t W = dst[0].width, H = dst[0].height;
-    if (W <= 0 || H <= 0)
-        return fail(STRATA_EINVAL, "engine: dst[0] is %lld×%lld; sizes must be positive", (long long)W, (long long)H);
-    for (int i = 0; i < nsrc; ++i) {
-        if (src[i].width != W || src[i].height != H)
-            return fail(STRATA_EINVAL, "engine: src[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
-                        (long long)src[i].width, (long long)src[i].height, (long long)W, (long long)H);
-        if (!src[i].resident && !src[i].raw && !src[i].read)
-            return fail(STRATA_EINVAL, "engine: src[%d] is a nil source", i);
-        if (!src[i].resident && src[i].raw && (rc = check_raw(src[i].raw, src[i].masked, "src", i))) return rc;
-        if (src[i].resident) {
-            if ((rc = check_raster("src", i, *src[i].resident))) return rc;
-            if (src[i].resident->width != W || src[i].resident->height != H)
-                return fail(STRATA_EINVAL, "engine: src[%d]'s raster does not match its size", i);
-        }
-    }
-    bool masked = false;
-    for (int i = 0; i < nsrc; ++i) masked = masked || src[i].masked;
-    for (int i = 0; i < ndst; ++i) {
-        if (dst[i].width != W || dst[i].height != H)
-            return fail(STRATA_EINVAL, "engine: dst[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
-                        (long long)dst[i].width, (long long)dst[i].height, (long long)W, (long long)H);
-        if (!dst[i].resident && !dst[i].raw && !dst[i].write)
-            return fail(STRATA_EINVAL, "engine: dst[%d] is a nil sink", i);
-        if (!dst[i].resident && dst[i].raw && (rc = check_raw(dst[i].raw, dst[i].masked, "dst", i))) return rc;
-        if (dst[i].resident && (rc = check_raster("dst", i, *dst[i].resident))) return rc;
-        if (masked && !dst[i].masked)
-            return fail(STRATA_EINVAL,
-                        "engine: a source is Masked but dst[%d] is not; its validity would be lost", i);
-    }
-    if ((rc = ensure_device())) return rc;
-    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
-    const int r = J.r, rmin = J.rmin;
-    const TileGrid g = tile_grid(W, H, opts);
-    const bool opts_cancel = opts && opts->cancel;
-
-    // Buffers for callback operands: inputs hold a tile grown by the
-    // radius, outputs a tile; a masked buffer's stride is a multiple of 64
-    // so every row's bits start a word (chunked.go:184-200).
-    const int64_t inW = std::min(W, g.tileW + 2 * r), inH = std::min(H, g.tileH + 2 * r);
-    auto round64 = [](int64_t n) { return (n + 63) & ~(int64_t)63; };
-    struct Buf {
-        int64_t stride = 0, cells = 0, words = 0;
-        float *hdata = nullptr;
-        uint64_t *hmask = nullptr;
-        float *ddata = nullptr;
-        uint64_t *dmask = nullptr;
-    };
-    // Two sets of buffers: while the device computes one tile, the host
-    // hands the previous tile's outputs to their sinks and reads the next
-    // tile (the write-behind of internal/exec/behind.go:54-192).
-    constexpr int kSlots = 2;
-    const int nbuf = nsrc + ndst;
-    std::vector<Buf> bufs(kSlots * nbuf);
-    hipEvent_t done[kSlots] = {nullptr, nullptr};
-    auto buf = [&](int slot, int i) -> Buf & { return bufs[slot * nbuf + i]; };
-    auto free_bufs = [&]() {
-        for (auto &b : bufs) {
-            if (b.hdata) (void)hipHostFree(b.hdata);
-            if (b.hmask) (void)hipHostFree(b.hmask);
-            if (b.ddata) (void)hipFree(b.ddata);
-            if (b.dmask) (void)hipFree(b.dmask);
-        }
-        for (auto &e : done)
-            if (e) (void)hipEventDestroy(e);
-    };
-    struct Guard {
-        std::function<void()> f;
-        ~Guard() { f(); }
-    };
-    auto alloc_buf = [&](Buf &b, int64_t w, int64_t h, bool m) -> int {
-        b.stride = m ? round64(w) : w;
-        b.cells = (h - 1) * b.stride + w;
-        b.words = m ? cdiv(b.cells, 64) : 0;
-        HIP_TRY(hipHostMalloc((void **)&b.hdata, (size_t)b.cells * 4, hipHostMallocDefault), "pinned buffer");
-        HIP_TRY(hipMalloc((void **)&b.ddata, (size_t)b.cells * 4), "tile buffer");
-        if (m) {
-            HIP_TRY(hipHostMalloc((void **)&b.hmask, (size_t)b.words * 8, hipHostMallocDefault), "pinned mask");
-            HIP_TRY(hipMalloc((void **)&b.dmask, (size_t)b.words * 8), "tile mask");
-        }
-        return STRATA_OK;
-    };
-    int status = STRATA_OK;
-    {
-        Guard guard{free_bufs};
-        strata_stats s{};
-        bool all_resident = true;
-        for (int i = 0; i < nsrc; ++i) all_resident = all_resident && src[i].resident;
-        for (int o = 0; o < ndst; ++o) all_resident = all_resident && dst[o].resident;
-        if (all_resident && !opts_cancel) {
-            // Device tile queue over resident operands with nothing to poll:
-            // every tile reads its halo straight from HBM and writes in place,
-            // so the queue is one launch over the raster (tiling never changes
-            // a result) and the tiles' Stats follow in closed form.
-            strata_raster sv = *src[0].resident, dv[kMaxDst]{};
-            if (sv.mem == STRATA_MEM_HOST) return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
-            if (!src[0].masked) sv.valid = nullptr;
-            for (int o = 0; o < ndst; ++o) {
-                dv[o] = *dst[o].resident;
-                if (dv[o].mem == STRATA_MEM_HOST)
-                    return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
-                if (!dst[o].masked) dv[o].valid = nullptr;
-            }
-            Operands op{&sv, 0, 0, {}, {}, {}, ndst, W, H};
-            for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
-            if ((rc = run_rect(J, op, Rect{0, 0, W, H}, st))) return rc;
-            for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, rmin, g, nsrc, ndst, tile_rect(g, t));
-            if (!(opts && opts->async)) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
-            add_stats(stats, s, nsrc, ndst);
-            g_err.clear();
-            return STRATA_OK;
-        }
-        for (int slot = 0; slot < kSlots; ++slot) {
-            for (int i = 0; i < nsrc; ++i)
-                if (!src[i].resident && (rc = alloc_buf(buf(slot, i), inW, inH, src[i].masked))) return rc;
-            for (int o = 0; o < ndst; ++o)
-                if (!dst[o].resident && (rc = alloc_buf(buf(slot, nsrc + o), g.tileW, g.tileH, dst[o].masked)))
-                    return rc;
-            HIP_TRY(hipEventCreateWithFlags(&done[slot], hipEventDisableTiming), "tile event");
-        }
-
-        // The tile whose outputs wait to be handed to the sinks.
-        struct Pending {
-            bool live = false;
-            int slot = 0;
-            Rect tr{};
-        } pend;
-        auto finish = [&](Pending &p) -> int {
-            if (!p.live) return STRATA_OK;
-            p.live = false;
-            HIP_TRY(hipEventSynchronize(done[p.slot]), "waiting for a tile");
-            const int64_t tw = p.tr.x1 - p.tr.x0, th = p.tr.y1 - p.tr.y0;
-            for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident) continue;
-                Buf &b = buf(p.slot, nsrc + o);
-                if (dst[o].raw) {
-                    int e = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, p.tr.x0, p.tr.y0, o);
-                    if (e) return e;
-                } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, p.tr.x0, p.tr.y0) != 0) {
-                    return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
-                                (long long)p.tr.x0, (long long)p.tr.y0);
-                }
-            }
-            tile_stats(s, r, rmin, g, nsrc, ndst, p.tr);
-            return STRATA_OK;
-        };
-
-        for (int64_t t = 0; t < g.tiles; ++t) {
-            if (cancelled(opts)) {
-                status = STRATA_CANCELLED;
-                break;
-            }
-            const int slot = (int)(t % kSlots);
-            const Rect tr = tile_rect(g, t);
-            const int64_t rx0 = std::max<int64_t>(0, tr.x0 - r), ry0 = std::max<int64_t>(0, tr.y0 - r);
-            const int64_t rx1 = std::min<int64_t>(W, tr.x1 + r), ry1 = std::min<int64_t>(H, tr.y1 + r);
-            const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
-
-            // The source window: resident (read in place) or read into the
-            // slot's pinned buffer and copied to the device. The slot's last
-            // tile was handed off by the previous iteration's finish().
-            strata_raster sv{};
-            int64_t sx = 0, sy = 0;
-            if (src[0].resident) {
-                sv = *src[0].resident;
-                if (sv.mem == STRATA_MEM_HOST) {
-                    status = fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
-                    break;
-                }
-                if (!src[0].masked) sv.valid = nullptr;
-            } else {
-                Buf &b = buf(slot, 0);
-                const int64_t ww = rx1 - rx0, hh = ry1 - ry0;
-                const int64_t cells = (hh - 1) * b.stride + ww;
-                if (src[0].raw) {
-                    // A raw file: rows read straight into the pinned buffer,
-                    // the validity derived from the fill on the device.
-                    if ((status = raw_read(*src[0].raw, W, b.hdata, ww, hh, b.stride, rx0, ry0, "src[0]"))) break;
-                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
-                            "tile in");
-                    if (b.dmask)
-                        HIP_TRY(launch_raw_validity(RawTile{b.ddata, b.stride, ww, hh, b.dmask}, src[0].raw->fill, st),
-                                "launching the fill validity kernel");
-                } else {
-                    if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
-                    if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
-                        status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
-                                      (long long)rx0, (long long)ry0);
-                        break;
-                    }
-                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
-                            "tile in");
-                    if (b.hmask)
-                        HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice,
-                                               st),
-                                "tile mask in");
-                }
-                sv.data = b.ddata;
-                sv.width = ww;
-                sv.height = hh;
-                sv.stride = b.stride;
-                sv.valid = b.hmask ? b.dmask : nullptr;
-                sv.valid_offset = 0;
-                sx = rx0;
-                sy = ry0;
-            }
-            strata_raster dv[kMaxDst]{};
-            Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
-            for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
-            bool bad_sink = false;
-            for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident) {
-                    dv[o] = *dst[o].resident;
-                    if (dv[o].mem == STRATA_MEM_HOST) bad_sink = true;
-                    if (!dst[o].masked) dv[o].valid = nullptr;
-                } else {
-                    Buf &b = buf(slot, nsrc + o);
-                    dv[o].data = b.ddata;
-                    dv[o].width = tw;
-                    dv[o].height = th;
-                    dv[o].stride = b.stride;
-                    dv[o].valid = b.dmask;
-                    dv[o].valid_offset = 0;
-                    op.dx[o] = tr.x0;
-                    op.dy[o] = tr.y0;
-                }
-            }
-            if (bad_sink) {
-                status = fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
-                break;
-            }
-            if ((rc = run_rect(J, op, tr, st))) return rc;
-            // Buffered outputs back to the host: a raw sink's fill first.
-            for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident) continue;
-                Buf &b = buf(slot, nsrc + o);
-                const int64_t cells = (th - 1) * b.stride + tw;
-                if (dst[o].raw && b.dmask)
-                    HIP_TRY(launch_raw_fill(RawTile{b.ddata, b.stride, tw, th, b.dmask}, dst[o].raw->fill, st),
-                            "launching the fill kernel");
-                HIP_TRY(hipMemcpyAsync(b.hdata, b.ddata, (size_t)cells * 4, hipMemcpyDeviceToHost, st), "tile out");
-                if (b.hmask && !dst[o].raw)
-                    HIP_TRY(hipMemcpyAsync(b.hmask, b.dmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyDeviceToHost, st),
-                            "tile mask out");
-            }
-            HIP_TRY(hipEventRecord(done[slot], st), "tile event");
-            // While the device runs this tile, the previous one goes out.
-            if ((status = finish(pend))) break;
-            pend.live = true;
-            pend.slot = slot;
-            pend.tr = tr;
-            // A cancellable call observes its flag once per finished tile.
-            if (opts_cancel && (status = finish(pend))) break;
-        }
-        // The claimed tile always completes (internal/exec/worker.go:112-181).
-        const int last = finish(pend);
-        if (status == STRATA_OK || status == STRATA_CANCELLED) status = last ? last : status;
-        HIP_TRY(hipStreamSynchronize(st), "synchronizing");
-        add_stats(stats, s, nsrc, ndst);
-    }
-    if (status == STRATA_OK) g_err.clear();
-    return status;
-}
-
 // ---------------------------------------------------------------------------
 // Reductions (internal/exec/reduce.go, reducechunked.go)
 // ---------------------------------------------------------------------------
@@ -1241,6 +1017,390 @@ void fold_band_stats(strata_stats &s, const Rect &b)
     s.cells += cells;
     s.bands += 1;
     s.kernel_read += cells * kBytesPerCell;
+}
+
+
+int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata_source *src, int nsrc,
+                    const strata_options *opts, strata_stats *stats)
+{
+    int rc = STRATA_OK;
+    if (!dst || !src) return fail(STRATA_EINVAL, "engine: nil operand list");
+    if (opts && (opts->tile_width < 0 || opts->tile_height < 0 || opts->workers < 0))
+        return fail(STRATA_EINVAL, "engine: negative Options");
+    const int64_t W = dst[0].width, H = dst[0].height;
+    if (W <= 0 || H <= 0)
+        return fail(STRATA_EINVAL, "engine: dst[0] is %lld×%lld; sizes must be positive", (long long)W, (long long)H);
+    for (int i = 0; i < nsrc; ++i) {
+        if (src[i].width != W || src[i].height != H)
+            return fail(STRATA_EINVAL, "engine: src[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)src[i].width, (long long)src[i].height, (long long)W, (long long)H);
+        if (!src[i].resident && !src[i].raw && !src[i].read)
+            return fail(STRATA_EINVAL, "engine: src[%d] is a nil source", i);
+        if (!src[i].resident && src[i].raw && (rc = check_raw(src[i].raw, src[i].masked, "src", i))) return rc;
+        if (src[i].resident) {
+            if ((rc = check_raster("src", i, *src[i].resident))) return rc;
+            if (src[i].resident->width != W || src[i].resident->height != H)
+                return fail(STRATA_EINVAL, "engine: src[%d]'s raster does not match its size", i);
+        }
+    }
+    bool masked = false;
+    for (int i = 0; i < nsrc; ++i) masked = masked || src[i].masked;
+    for (int i = 0; i < ndst; ++i) {
+        if (dst[i].width != W || dst[i].height != H)
+            return fail(STRATA_EINVAL, "engine: dst[%d] is %lld×%lld, dst[0] is %lld×%lld", i,
+                        (long long)dst[i].width, (long long)dst[i].height, (long long)W, (long long)H);
+        if (!dst[i].resident && !dst[i].raw && !dst[i].write && !dst[i].fold)
+            return fail(STRATA_EINVAL, "engine: dst[%d] is a nil sink", i);
+        if (dst[i].fold && (rc = check_reduce(dst[i].fold_what, nullptr))) return rc;
+        if (!dst[i].resident && dst[i].raw && (rc = check_raw(dst[i].raw, dst[i].masked, "dst", i))) return rc;
+        if (dst[i].resident && (rc = check_raster("dst", i, *dst[i].resident))) return rc;
+        if (masked && !dst[i].masked)
+            return fail(STRATA_EINVAL,
+                        "engine: a source is Masked but dst[%d] is not; its validity would be lost", i);
+    }
+    for (int o = 0; o < ndst; ++o)
+        if (dst[o].fold) memset(dst[o].fold, 0, sizeof *dst[o].fold);
+    if ((rc = ensure_device())) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    const int r = J.r, rmin = J.rmin;
+    const TileGrid g = tile_grid(W, H, opts);
+    const bool opts_cancel = opts && opts->cancel;
+
+    // The outputs' folds (graph/sinks.go:17-71): one set of bins per
+    // folded output, accumulated over every tile its sink is handed. A
+    // fold-only sink writes nothing: its tiles live in device buffers.
+    FoldScratch folds[kMaxDst];
+    bool fold_first[kMaxDst];
+    auto fold_only = [&](int o) { return dst[o].fold && !dst[o].resident && !dst[o].raw && !dst[o].write; };
+    for (int o = 0; o < ndst; ++o) {
+        fold_first[o] = true;
+        folds[o].st = st;
+        if (dst[o].fold)
+            HIP_TRY(hipMallocAsync((void **)&folds[o].dev, FoldScratch::bytes(), st), "fold scratch");
+    }
+    // Folds output o's cells of rect rc (raster coordinates), held in d
+    // whose cell (0, 0) is raster cell (dx, dy): exec.Reduce over the
+    // window the sink is handed, valid cells only when the sink is masked.
+    auto fold_tile = [&](int o, const strata_raster &d, int64_t dx, int64_t dy, const Rect &rc) -> int {
+        strata_raster v = d;
+        const int64_t off = (rc.y0 - dy) * d.stride + (rc.x0 - dx);
+        v.data = d.data + off;
+        v.valid_offset = d.valid_offset + off;
+        v.width = rc.x1 - rc.x0;
+        v.height = rc.y1 - rc.y0;
+        HIP_TRY(launch_fold(fold_args(v, dst[o].fold_what, dst[o].masked && d.valid != nullptr), folds[o].bins(),
+                            folds[o].parts(), folds[o].total(), fold_first[o], st),
+                "launching an output's fold");
+        fold_first[o] = false;
+        return STRATA_OK;
+    };
+    // The combined partials into the callers' structs (waits for the stream).
+    auto fold_store = [&]() -> int {
+        for (int o = 0; o < ndst; ++o) {
+            if (!dst[o].fold) continue;
+            if (fold_first[o]) {  // no tile: the empty fold
+                dst[o].fold->min = INFINITY;
+                dst[o].fold->max = -INFINITY;
+                continue;
+            }
+            int e = fold_read(folds[o], dst[o].fold_what, dst[o].fold);
+            if (e) return e;
+        }
+        return STRATA_OK;
+    };
+
+    // Buffers for callback operands: inputs hold a tile grown by the
+    // radius, outputs a tile; a masked buffer's stride is a multiple of 64
+    // so every row's bits start a word (chunked.go:184-200).
+    const int64_t inW = std::min(W, g.tileW + 2 * r), inH = std::min(H, g.tileH + 2 * r);
+    auto round64 = [](int64_t n) { return (n + 63) & ~(int64_t)63; };
+    struct Buf {
+        int64_t stride = 0, cells = 0, words = 0;
+        float *hdata = nullptr;
+        uint64_t *hmask = nullptr;
+        float *ddata = nullptr;
+        uint64_t *dmask = nullptr;
+    };
+    // Two sets of buffers: while the device computes one tile, the host
+    // hands the previous tile's outputs to their sinks and reads the next
+    // tile (the write-behind of internal/exec/behind.go:54-192).
+    constexpr int kSlots = 2;
+    const int nbuf = nsrc + ndst;
+    std::vector<Buf> bufs(kSlots * nbuf);
+    hipEvent_t done[kSlots] = {nullptr, nullptr};
+    auto buf = [&](int slot, int i) -> Buf & { return bufs[slot * nbuf + i]; };
+    auto free_bufs = [&]() {
+        for (auto &b : bufs) {
+            if (b.hdata) (void)hipHostFree(b.hdata);
+            if (b.hmask) (void)hipHostFree(b.hmask);
+            if (b.ddata) (void)hipFree(b.ddata);
+            if (b.dmask) (void)hipFree(b.dmask);
+        }
+        for (auto &e : done)
+            if (e) (void)hipEventDestroy(e);
+    };
+    struct Guard {
+        std::function<void()> f;
+        ~Guard() { f(); }
+    };
+    auto alloc_buf = [&](Buf &b, int64_t w, int64_t h, bool m, bool host = true) -> int {
+        b.stride = m ? round64(w) : w;
+        b.cells = (h - 1) * b.stride + w;
+        b.words = m ? cdiv(b.cells, 64) : 0;
+        if (host) HIP_TRY(hipHostMalloc((void **)&b.hdata, (size_t)b.cells * 4, hipHostMallocDefault), "pinned buffer");
+        HIP_TRY(hipMalloc((void **)&b.ddata, (size_t)b.cells * 4), "tile buffer");
+        if (m) {
+            if (host)
+                HIP_TRY(hipHostMalloc((void **)&b.hmask, (size_t)b.words * 8, hipHostMallocDefault), "pinned mask");
+            HIP_TRY(hipMalloc((void **)&b.dmask, (size_t)b.words * 8), "tile mask");
+        }
+        return STRATA_OK;
+    };
+    int status = STRATA_OK;
+    {
+        Guard guard{free_bufs};
+        strata_stats s{};
+        bool all_resident = true, any_fold_only = false;
+        for (int i = 0; i < nsrc; ++i) all_resident = all_resident && src[i].resident;
+        for (int o = 0; o < ndst; ++o) {
+            all_resident = all_resident && (dst[o].resident || fold_only(o));
+            any_fold_only = any_fold_only || fold_only(o);
+        }
+        if (all_resident && !opts_cancel) {
+            // Device tile queue over resident operands with nothing to poll:
+            // every tile reads its halo straight from HBM and writes in place,
+            // so the queue is one launch over the raster (tiling never changes
+            // a result) and the tiles' Stats follow in closed form. Outputs
+            // that are only folded go through a device buffer of a few
+            // hundred rows instead, one step of rows at a time, each step
+            // folded while the cache still holds it.
+            strata_raster sv = *src[0].resident, dv[kMaxDst]{};
+            if (sv.mem == STRATA_MEM_HOST) return fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+            if (!src[0].masked) sv.valid = nullptr;
+            int64_t step = H;
+            if (any_fold_only) {
+                const int64_t bytes = (int64_t)get_tuning().fold_stripe_mb << 20;
+                step = std::min(H, std::max<int64_t>(32, bytes / (4 * W) / 32 * 32));
+            }
+            // Stream-ordered buffers: the pool reuses them from call to call.
+            struct AsyncBufs {
+                hipStream_t st;
+                std::vector<void *> p;
+                ~AsyncBufs()
+                {
+                    for (void *q : p) (void)hipFreeAsync(q, st);
+                }
+            } abufs{st, {}};
+            hipMemPool_t pool = nullptr;
+            if (any_fold_only && (rc = scratch_pool(&pool))) return rc;
+            for (int o = 0; o < ndst; ++o) {
+                if (fold_only(o)) {
+                    const int64_t cells = step * W;
+                    float *data = nullptr;
+                    uint64_t *mask = nullptr;
+                    HIP_TRY(hipMallocFromPoolAsync((void **)&data, (size_t)cells * 4, pool, st), "fold buffer");
+                    abufs.p.push_back(data);
+                    if (dst[o].masked) {
+                        HIP_TRY(hipMallocFromPoolAsync((void **)&mask, (size_t)cdiv(cells, 64) * 8, pool, st),
+                                "fold mask");
+                        abufs.p.push_back(mask);
+                    }
+                    dv[o].data = data;
+                    dv[o].width = W;
+                    dv[o].height = step;
+                    dv[o].stride = W;
+                    dv[o].valid = mask;
+                    dv[o].mem = STRATA_MEM_DEVICE;
+                    continue;
+                }
+                dv[o] = *dst[o].resident;
+                if (dv[o].mem == STRATA_MEM_HOST)
+                    return fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                if (!dst[o].masked) dv[o].valid = nullptr;
+            }
+            for (int64_t y0 = 0; y0 < H; y0 += step) {
+                const Rect sr{0, y0, W, std::min(H, y0 + step)};
+                Operands op{&sv, 0, 0, {}, {}, {}, ndst, W, H};
+                for (int o = 0; o < ndst; ++o) {
+                    op.dst[o] = &dv[o];
+                    if (fold_only(o)) op.dy[o] = y0;
+                }
+                if ((rc = run_rect(J, op, sr, st))) return rc;
+                for (int o = 0; o < ndst; ++o)
+                    if (dst[o].fold && (rc = fold_tile(o, dv[o], op.dx[o], op.dy[o], sr))) return rc;
+            }
+            for (int64_t t = 0; t < g.tiles; ++t) tile_stats(s, r, rmin, g, nsrc, ndst, tile_rect(g, t));
+            bool any_fold = false;
+            for (int o = 0; o < ndst; ++o) any_fold = any_fold || dst[o].fold;
+            if (any_fold) {
+                if ((rc = fold_store())) return rc;
+            } else if (!(opts && opts->async)) {
+                HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+            }
+            add_stats(stats, s, nsrc, ndst);
+            g_err.clear();
+            return STRATA_OK;
+        }
+        for (int slot = 0; slot < kSlots; ++slot) {
+            for (int i = 0; i < nsrc; ++i)
+                if (!src[i].resident && (rc = alloc_buf(buf(slot, i), inW, inH, src[i].masked))) return rc;
+            for (int o = 0; o < ndst; ++o)
+                if (!dst[o].resident &&
+                    (rc = alloc_buf(buf(slot, nsrc + o), g.tileW, g.tileH, dst[o].masked, !fold_only(o))))
+                    return rc;
+            HIP_TRY(hipEventCreateWithFlags(&done[slot], hipEventDisableTiming), "tile event");
+        }
+
+        // The tile whose outputs wait to be handed to the sinks.
+        struct Pending {
+            bool live = false;
+            int slot = 0;
+            Rect tr{};
+        } pend;
+        auto finish = [&](Pending &p) -> int {
+            if (!p.live) return STRATA_OK;
+            p.live = false;
+            HIP_TRY(hipEventSynchronize(done[p.slot]), "waiting for a tile");
+            const int64_t tw = p.tr.x1 - p.tr.x0, th = p.tr.y1 - p.tr.y0;
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident || fold_only(o)) continue;
+                Buf &b = buf(p.slot, nsrc + o);
+                if (dst[o].raw) {
+                    int e = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, p.tr.x0, p.tr.y0, o);
+                    if (e) return e;
+                } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, p.tr.x0, p.tr.y0) != 0) {
+                    return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): sink error", o,
+                                (long long)p.tr.x0, (long long)p.tr.y0);
+                }
+            }
+            tile_stats(s, r, rmin, g, nsrc, ndst, p.tr);
+            return STRATA_OK;
+        };
+
+        for (int64_t t = 0; t < g.tiles; ++t) {
+            if (cancelled(opts)) {
+                status = STRATA_CANCELLED;
+                break;
+            }
+            const int slot = (int)(t % kSlots);
+            const Rect tr = tile_rect(g, t);
+            const int64_t rx0 = std::max<int64_t>(0, tr.x0 - r), ry0 = std::max<int64_t>(0, tr.y0 - r);
+            const int64_t rx1 = std::min<int64_t>(W, tr.x1 + r), ry1 = std::min<int64_t>(H, tr.y1 + r);
+            const int64_t tw = tr.x1 - tr.x0, th = tr.y1 - tr.y0;
+
+            // The source window: resident (read in place) or read into the
+            // slot's pinned buffer and copied to the device. The slot's last
+            // tile was handed off by the previous iteration's finish().
+            strata_raster sv{};
+            int64_t sx = 0, sy = 0;
+            if (src[0].resident) {
+                sv = *src[0].resident;
+                if (sv.mem == STRATA_MEM_HOST) {
+                    status = fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
+                    break;
+                }
+                if (!src[0].masked) sv.valid = nullptr;
+            } else {
+                Buf &b = buf(slot, 0);
+                const int64_t ww = rx1 - rx0, hh = ry1 - ry0;
+                const int64_t cells = (hh - 1) * b.stride + ww;
+                if (src[0].raw) {
+                    // A raw file: rows read straight into the pinned buffer,
+                    // the validity derived from the fill on the device.
+                    if ((status = raw_read(*src[0].raw, W, b.hdata, ww, hh, b.stride, rx0, ry0, "src[0]"))) break;
+                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
+                            "tile in");
+                    if (b.dmask)
+                        HIP_TRY(launch_raw_validity(RawTile{b.ddata, b.stride, ww, hh, b.dmask}, src[0].raw->fill, st),
+                                "launching the fill validity kernel");
+                } else {
+                    if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
+                    if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
+                        status = fail(STRATA_EIO, "engine: reading src[0] at (%lld, %lld): source error",
+                                      (long long)rx0, (long long)ry0);
+                        break;
+                    }
+                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
+                            "tile in");
+                    if (b.hmask)
+                        HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice,
+                                               st),
+                                "tile mask in");
+                }
+                sv.data = b.ddata;
+                sv.width = ww;
+                sv.height = hh;
+                sv.stride = b.stride;
+                sv.valid = b.hmask ? b.dmask : nullptr;
+                sv.valid_offset = 0;
+                sx = rx0;
+                sy = ry0;
+            }
+            strata_raster dv[kMaxDst]{};
+            Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
+            for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
+            bool bad_sink = false;
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident) {
+                    dv[o] = *dst[o].resident;
+                    if (dv[o].mem == STRATA_MEM_HOST) bad_sink = true;
+                    if (!dst[o].masked) dv[o].valid = nullptr;
+                } else {
+                    Buf &b = buf(slot, nsrc + o);
+                    dv[o].data = b.ddata;
+                    dv[o].width = tw;
+                    dv[o].height = th;
+                    dv[o].stride = b.stride;
+                    dv[o].valid = b.dmask;
+                    dv[o].valid_offset = 0;
+                    op.dx[o] = tr.x0;
+                    op.dy[o] = tr.y0;
+                }
+            }
+            if (bad_sink) {
+                status = fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
+                break;
+            }
+            if ((rc = run_rect(J, op, tr, st))) return rc;
+            // Folds see the tile before its sink does (graph/sinks.go:32-38).
+            for (int o = 0; o < ndst; ++o)
+                if (dst[o].fold && (rc = fold_tile(o, dv[o], op.dx[o], op.dy[o], tr))) return rc;
+            // Buffered outputs back to the host: a raw sink's fill first.
+            for (int o = 0; o < ndst; ++o) {
+                if (dst[o].resident || fold_only(o)) continue;
+                Buf &b = buf(slot, nsrc + o);
+                const int64_t cells = (th - 1) * b.stride + tw;
+                if (dst[o].raw && b.dmask)
+                    HIP_TRY(launch_raw_fill(RawTile{b.ddata, b.stride, tw, th, b.dmask}, dst[o].raw->fill, st),
+                            "launching the fill kernel");
+                HIP_TRY(hipMemcpyAsync(b.hdata, b.ddata, (size_t)cells * 4, hipMemcpyDeviceToHost, st), "tile out");
+                if (b.hmask && !dst[o].raw)
+                    HIP_TRY(hipMemcpyAsync(b.hmask, b.dmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyDeviceToHost, st),
+                            "tile mask out");
+            }
+            HIP_TRY(hipEventRecord(done[slot], st), "tile event");
+            // While the device runs this tile, the previous one goes out.
+            if ((status = finish(pend))) break;
+            pend.live = true;
+            pend.slot = slot;
+            pend.tr = tr;
+            // A cancellable call observes its flag once per finished tile.
+            if (opts_cancel && (status = finish(pend))) break;
+        }
+        // The claimed tile always completes (internal/exec/worker.go:112-181).
+        const int last = finish(pend);
+        if (status == STRATA_OK || status == STRATA_CANCELLED) status = last ? last : status;
+        HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+        add_stats(stats, s, nsrc, ndst);
+        if (status == STRATA_OK) {
+            status = fold_store();
+        }
+        if (status != STRATA_OK)
+            for (int o = 0; o < ndst; ++o)
+                if (dst[o].fold) memset(dst[o].fold, 0, sizeof *dst[o].fold);
+    }
+    if (status == STRATA_OK) g_err.clear();
+    return status;
 }
 
 } // namespace

@@ -741,6 +741,146 @@ def _read_callback(ctx, s: RasterSource, j: int, errors: List[Exception]):
     return _lib.READ_FN(read)
 
 
+# -- statistics folded into the pass (graph/sinks.go) ------------------------------------
+
+def _go_min(a: float, b: float) -> float:
+    """Go's builtin min of two float32s: NaN if either is, -0 < +0."""
+    if a != a or b != b:
+        return float("nan")
+    if a == b:
+        return a if np.signbit(a) else b
+    return min(a, b)
+
+
+def _go_max(a: float, b: float) -> float:
+    if a != a or b != b:
+        return float("nan")
+    if a == b:
+        return b if np.signbit(a) else a
+    return max(a, b)
+
+
+def _set_limbs(limbs, x: int) -> None:
+    n = len(limbs)
+    x &= (1 << (64 * n)) - 1  # two's complement over n limbs
+    for i in range(n):
+        limbs[i] = (x >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
+
+
+def _empty_fold() -> "_lib.Fold":
+    f = _lib.Fold()
+    f.min, f.max = float("inf"), float("-inf")
+    return f
+
+
+def _combine_folds(a: "_lib.Fold", b: "_lib.Fold") -> None:
+    """Folds partial b into a (summary.Reducer.Combine, internal/summary/
+    summary.go:62-75): counts and tallies add, the extremes take Go's min
+    and max, the exact sums add as integers."""
+    if b.count == 0:
+        return
+    if a.count == 0:
+        a.min, a.max = b.min, b.max
+    else:
+        a.min, a.max = _go_min(a.min, b.min), _go_max(a.max, b.max)
+    a.count += b.count
+    a.nan += b.nan
+    a.pos_inf += b.pos_inf
+    a.neg_inf += b.neg_inf
+    a.not_neg_zero = 1 if (a.not_neg_zero or b.not_neg_zero) else 0
+    _set_limbs(a.sum, _lib.limbs_to_int(a.sum) + _lib.limbs_to_int(b.sum))
+    _set_limbs(a.sumsq, _lib.limbs_to_int(a.sumsq) + _lib.limbs_to_int(b.sumsq))
+
+
+STATS_FOLD = _lib.REDUCE_MINMAX | _lib.REDUCE_MOMENTS  # what reduce.Stats folds
+
+
+class TeeSink(RasterSink):
+    """graph's teeSink (graph/sinks.go:17-71): every window goes to each of
+    sinks, and is first folded into a summary of the whole output (the
+    partial reduce.Stats combines; what selects the reduction, 0 for
+    none). With no sinks nothing is written: statistics alone write no
+    raster.
+
+    ProcessChunked and the fan-out form fold on the device, inside the pass
+    that computes the output, before any sink sees the tile; WriteWindow
+    called directly folds the window with Reduce. Fold() is the combined
+    partial of every window so far; reduce.Folded finishes it."""
+
+    def __init__(self, width: int, height: int, masked: bool, sinks: Sequence[RasterSink] = (),
+                 what: int = STATS_FOLD):
+        if width <= 0 or height <= 0:
+            raise EnginePanic(f"engine: tee sink dimensions must be positive, got {width}×{height}")
+        for i, t in enumerate(sinks):
+            if t is None:
+                raise EnginePanic(f"engine: tee sink's sink {i} is nil")
+            if t.Size() != (width, height):
+                raise EnginePanic(f"engine: tee sink is {width}×{height}, its sink {i} is {t.Size()}")
+        self.w, self.h, self.masked = width, height, bool(masked)
+        self.sinks = list(sinks)
+        self.what = what
+        self._fold = _empty_fold()
+        self._mu = threading.Lock()
+
+    def Size(self):
+        return self.w, self.h
+
+    def Masked(self) -> bool:
+        return self.masked
+
+    def Fold(self) -> "_lib.Fold":
+        return self._fold
+
+    def _add(self, f: "_lib.Fold") -> None:
+        with self._mu:
+            _combine_folds(self._fold, f)
+
+    def WriteWindow(self, ctx, src: Float32Raster, x: int, y: int):
+        # Fold first: a sink may overwrite the Data of invalid cells.
+        if self.what:
+            f, err = Reduce(ctx, [src], self.what, Options(Workers=1))
+            if err is not None:
+                return err
+            self._add(f)
+        for t in self.sinks:
+            err = t.WriteWindow(ctx, src, x, y)
+            if err is not None:
+                return err
+        return None
+
+
+class _TeeWrites(RasterSink):
+    """A tee's writes alone (its fold runs on the device)."""
+
+    def __init__(self, tee: TeeSink):
+        self.tee = tee
+
+    def Size(self):
+        return self.tee.Size()
+
+    def Masked(self) -> bool:
+        return self.tee.Masked()
+
+    def WriteWindow(self, ctx, src: Float32Raster, x: int, y: int):
+        for t in self.tee.sinks:
+            err = t.WriteWindow(ctx, src, x, y)
+            if err is not None:
+                return err
+        return None
+
+
+def _sink_target(d: RasterSink) -> Optional[RasterSink]:
+    """What the library writes for sink d: d itself, or for a tee its one
+    sink, a writer over its sinks, or None when it only folds."""
+    if not isinstance(d, TeeSink):
+        return d
+    if len(d.sinks) == 1:
+        return d.sinks[0]
+    if not d.sinks:
+        return None if d.what else _TeeWrites(d)
+    return _TeeWrites(d)
+
+
 def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[RasterSource], k: KernelDesc,
                    opts: Optional[Options] = None, *, async_: bool = False) -> Optional[Exception]:
     """exec.ProcessChunked: tiles read with their halo, computed on the
@@ -769,12 +909,13 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
             raise EnginePanic(f"engine: dst[{i}] is {sw}×{sh}, dst[0] is {w}×{h}")
         if masked and not d.Masked():
             raise EnginePanic(f"engine: a source is Masked but dst[{i}] is not; its validity would be lost")
-    for i, d in enumerate(dst):
+    targets = [_sink_target(d) for d in dst]
+    for i, d in enumerate(targets):
         if not isinstance(d, MemorySink):
             continue
         a = d.r
         for j in range(i + 1, len(dst)):
-            o = dst[j]
+            o = targets[j]
             if isinstance(o, MemorySink) and (data_spans(a, o.r) or _words_meet(a, o.r)):
                 raise EnginePanic(f"engine: memory sinks dst[{i}] and dst[{j}] share memory")
         for j, s in enumerate(src):
@@ -827,10 +968,18 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
             keep.append(fn)
             c.read = fn
     cdst = (_lib.Sink * len(dst))()
+    folds = {}  # dst index -> the device's partial of that tee
     for j, d in enumerate(dst):
         sw, sh = d.Size()
         c = cdst[j]
         c.width, c.height, c.masked = sw, sh, 1 if d.Masked() else 0
+        if isinstance(d, TeeSink) and d.what:
+            folds[j] = _lib.Fold()
+            c.fold = ctypes.pointer(folds[j])
+            c.fold_what = d.what
+        d = targets[j]
+        if d is None:
+            continue
         res = _resident(d)
         raw = _raw(d)
         if res is not None:
@@ -845,7 +994,7 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
             keep.append(fn)
             c.write = fn
     kc, taps = k.to_c()
-    rasters = [x.r for x in list(src) + list(dst) if isinstance(x, (MemorySource, MemorySink))]
+    rasters = [x.r for x in list(src) + targets if isinstance(x, (MemorySource, MemorySink))]
     co = _c_options(opts, ctx if not async_ else None, _stream_handle(rasters), async_)
     st = _lib.Stats()
     if isinstance(k, FanOut):
@@ -857,6 +1006,9 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
     del keep, taps
     if opts.Stats is not None:
         opts.Stats._add_c(st)
+    if rc == 0:
+        for j, f in folds.items():
+            dst[j]._add(f)
     if errors and not isinstance(errors[0], SourceError):
         raise errors[0]
     if rc == _lib.EIO:

@@ -278,3 +278,11 @@ def Stats(src: Float32Raster) -> Summary:
     and Max of the valid cells from one pass."""
     s, _ = StatsTiled(None, src, _plain)
     return s
+
+
+def Folded(tee) -> Summary:
+    """The summary an engine.TeeSink folded (graph/sinks.go:68-71,
+    folder.summary): reduce.Stats of every window the tee was handed,
+    folded in the pass that computed them. A tee whose reduction is not
+    the full Stats fold gives only the fields its reduction holds."""
+    return _summary(tee.Fold())

@@ -210,6 +210,39 @@ def test_wide_spread_with_specials(gpu, seed):
     same_summary(rd.Stats(d), sw)
 
 
+def window_values(rng, n, field0, spread):
+    """n float32s with random signs and full 24-bit significands whose
+    exponent fields lie in [field0, field0 + spread) (field 0 is a
+    subnormal), a few of them zero."""
+    fields = rng.integers(field0, field0 + spread, n).astype(np.uint32)
+    mant = rng.integers(0, 1 << 23, n, dtype=np.uint32)
+    sign = rng.integers(0, 2, n, dtype=np.uint32) << np.uint32(31)
+    vals = (sign | (fields << np.uint32(23)) | mant).view(np.float32)
+    vals[rng.integers(0, n, 5)] = 0.0
+    return vals
+
+
+@pytest.mark.parametrize("field0", [0, 120, 244, 235])
+@pytest.mark.parametrize("spread", [10, 11, 12, 19, 20, 21])
+def test_exponent_windows(gpu, field0, spread):
+    """Steps whose exponents fill the fold's fast window exactly (11 bins
+    with squares, 20 without), or overflow it by one, at the bottom,
+    middle and top of the exponent range: the exact integers are the
+    oracle's either way."""
+    if field0 + spread > 255:
+        pytest.skip("past the largest exponent")
+    rng = np.random.default_rng(1000 * field0 + spread)
+    W, H = 2048, 3
+    vals = window_values(rng, W * H, field0, spread)
+    r = Float32Raster(vals, W, H, W, None, 0)
+    fw, sw = oracle_of(r)
+    d = to_device(r, gpu)
+    for what in (_lib.REDUCE_SUM, _lib.REDUCE_MINMAX | _lib.REDUCE_MOMENTS):
+        f, err = engine.Reduce(None, [d], what)
+        assert err is None
+        check_fold(f, fw, what)
+
+
 def test_invalid_data_never_read(gpu):
     """TestStatsInvalidDataNeverRead (reduce/stats_test.go:265-286)."""
     r = dem_like(7, 200, 30, True)
