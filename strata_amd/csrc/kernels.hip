@@ -573,7 +573,10 @@ constexpr int focal_lds_floats()
 // One kTileX x kTileY output tile at (tx0, ty0) of the region.
 // For terrain.Surface through the fit (P.op STRATA_OP_SURFACE), dset holds
 // one view per wanted product, in output order.
-template <int KIND, int R, class TAPS>
+// UNROLL: stage and run the column pass with compile-time trip counts (all
+// loads in flight at once). The batch's kernel, which inlines every kind,
+// keeps the rolled loops: unrolled, it needs several times the registers.
+template <int KIND, int R, class TAPS, bool UNROLL = true>
 __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps, const Region &rg,
                                             const SrcView &s, const DstView &d0, const DstView &d1, int64_t tx0,
                                             int64_t ty0, float *lds, const DstView *dset = nullptr)
@@ -586,10 +589,43 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
     float(*colv)[CY][LW] = reinterpret_cast<float(*)[CY][LW]>(lds + LH * LW);
 
     const int tid = threadIdx.x;
+    // Staging: every load of the thread is issued before any LDS write
+    // (the trip count is a compile-time constant), and a tile whose staged
+    // window lies inside the buffer takes unchecked loads.
+    constexpr int kStage = (LH * LW + kTileThreads - 1) / kTileThreads;
+    if constexpr (!UNROLL) {
+        __syncthreads();  // the LDS may still be read by the previous tile
+        for (int i = tid; i < LH * LW; i += kTileThreads) {
+            const int row = i / LW, col = i - row * LW;
+            lds[i] = load_cell(s, tx0 - R + col, ty0 - R + row);
+        }
+    } else {
+    float sv[kStage];
+    {
+        const int64_t bx = tx0 - R, by = ty0 - R;
+        if (bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1) {
+            const STRATA_GLOBAL float *base = gp(s.p) + (by - s.sy) * s.stride + (bx - s.sx);
+#pragma unroll
+            for (int k = 0; k < kStage; ++k) {
+                const int i = min(tid + k * kTileThreads, LH * LW - 1);
+                const int row = i / LW, col = i - row * LW;
+                sv[k] = base[(int64_t)row * s.stride + col];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kStage; ++k) {
+                const int i = min(tid + k * kTileThreads, LH * LW - 1);
+                const int row = i / LW, col = i - row * LW;
+                sv[k] = load_cell(s, bx + col, by + row);
+            }
+        }
+    }
     __syncthreads();  // the LDS may still be read by the previous tile of a batch
-    for (int i = tid; i < LH * LW; i += kTileThreads) {
-        int row = i / LW, col = i - row * LW;
-        tile[row][col] = load_cell(s, tx0 - R + col, ty0 - R + row);
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+        const int i = tid + k * kTileThreads;
+        if (i < LH * LW) lds[i] = sv[k];
+    }
     }
     if constexpr (KIND == FK_CORRELATE && R > 2) {
         constexpr int WP = weight_pitch<R>();
@@ -657,8 +693,10 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
 #pragma unroll
         for (int o = 0; o < kRowsPerWave; ++o) out0[o] = acc[o];
     } else {
-        // Column pass for the tile's rows over the staged columns.
-        for (int i = tid; i < kTileY * LW; i += kTileThreads) {
+        // Column pass for the tile's rows over the staged columns. Unrolled
+        // (a compile-time trip count), the LDS reads of several cells are in
+        // flight together.
+        auto col_cell = [&](int i) {
             const int rr = i / LW, cc = i - rr * LW;
             if constexpr (KIND == FK_SEPARABLE) {
                 // ColumnCorrelate with the col taps (focalrow.go:232-242).
@@ -700,6 +738,16 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
                     colv[2][rr][cc] = ct;
                 }
             }
+        };
+        if constexpr (UNROLL) {
+            constexpr int kColIters = (kTileY * LW + kTileThreads - 1) / kTileThreads;
+#pragma unroll
+            for (int k = 0; k < kColIters; ++k) {
+                const int i = tid + k * kTileThreads;
+                if (i < kTileY * LW) col_cell(i);
+            }
+        } else {
+            for (int i = tid; i < kTileY * LW; i += kTileThreads) col_cell(i);
         }
         __syncthreads();
 #pragma unroll
@@ -1313,10 +1361,11 @@ __device__ __forceinline__ void batch_focal_tiles(const FocalArgs &a, const Batc
     const PtrTaps taps{nullptr};
     const int64_t tx0 = t.rg.ox + (lu % t.blocks_x) * kTileX;
     const int64_t ty = t.rg.oy + (lu / t.blocks_x) * t.rows;
+#pragma unroll 1
     for (int k = 0; k < kFocalTilesPerUnit; ++k) {
         const int64_t ty0 = ty + k * kTileY;
         if (ty0 >= t.rg.oy + t.rg.h) break;
-        focal_block<KIND, R>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds);
+        focal_block<KIND, R, PtrTaps, false>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds);
     }
 }
 
