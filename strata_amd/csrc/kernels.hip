@@ -103,6 +103,7 @@ struct HornArgs {
 // each of the two columns), which three output rows share.
 struct Row {
     float c0, c1, l, r, hd0, hd1;
+    bool odd;  // kSweepAnyParity: the row's cells at x - 1, x were loaded as a pair
 };
 
 // Correctly rounded sqrtf for x = +0 or x in [2^-96, FLT_MAX]: the hardware
@@ -280,15 +281,31 @@ struct NoPost {
     __device__ float operator()(float v) const { return v; }
 };
 
-// PAIRS: every pair of the block lies inside the buffer and the region, at
-// 8-byte-aligned addresses, so rows move as dwordx2 loads and stores. `post`
+// MODE (wave-uniform, chosen per block or unit):
+//   kSweepChecked   every cell loaded and stored on its own, columns clamped
+//                   into the buffer and stores kept inside the region;
+//   kSweepPairs     every pair of the block lies inside the buffer and the
+//                   region, at 8-byte-aligned addresses, so rows move as
+//                   dwordx2 loads and stores;
+//   kSweepAnyParity the block's columns lie inside the buffer and the
+//                   region, at any alignment: each row (uniform per row)
+//                   loads and stores aligned pairs, one cell to the left
+//                   when its first cell is at an odd 4-byte address, and
+//                   the lanes' cells are formed with the same DPP moves.
+//                   A lane 0 pair that starts one cell before the buffer
+//                   lies in the 8-byte block of the buffer's first cell
+//                   and feeds only an edge output; likewise lane 63's pair
+//                   that ends one cell past it.
+// `post`
 // maps every stored value (edge NaNs included) of the first output.
 // tid: the lane's index among the lanes sharing columns from xb (a block's
 // 256 or a wave's 64); each lane takes columns xb + 2*tid and xb + 2*tid + 1.
 // For STRATA_OP_SURFACE, dset holds one view per wanted product (a.flags'
 // STRATA_SURF_* bits, in output order) and d0, d1 are unused.
 // For kOpHornFan, dset holds output o's view and fan its op.
-template <int OP, bool PAIRS, class Post = NoPost>
+constexpr int kSweepChecked = 0, kSweepPairs = 1, kSweepAnyParity = 2;
+
+template <int OP, int MODE, class Post = NoPost>
 __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const SrcView &s, const DstView &d0,
                                       const DstView &d1, int64_t xb, int64_t y0, int64_t y1, uint32_t tid,
                                       const Post &post = Post(), const DstView *dset = nullptr,
@@ -317,35 +334,85 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     // Source rows are clamped into the buffer in 32 bits relative to y0.
     const int32_t rlo = (int32_t)min(max(s.ly0 - y0, (int64_t)-2), (int64_t)0);
     const int32_t rhi = (int32_t)min(max(s.ly1 - 1 - y0, (int64_t)-2), (int64_t)4097);
+    // kSweepAnyParity: byte offsets of the block's first cell from a row's
+    // start, in the source and in an output (uniform).
+    const uint32_t lbase = 4u * (uint32_t)(xb - s.sx), sbase = 4u * (uint32_t)(xb - rg.ox);
     auto load = [&](Row &row, int32_t u) {  // row y0 + u
         const int32_t uc = min(max(u, rlo), rhi);
         const crow_t rp = reinterpret_cast<crow_t>(gp(s.p) + (y0 + uc - s.sy) * s.stride);
-        float c0, c1, h = 0.0f;
-        if constexpr (PAIRS) {
+        float c0, c1, h = 0.0f, h2 = 0.0f;
+        row.odd = false;
+        if constexpr (MODE == kSweepPairs) {
             const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0);
             c0 = v.x;
             c1 = v.y;
+        } else if constexpr (MODE == kSweepAnyParity) {
+            row.odd = ((reinterpret_cast<uintptr_t>(rp) + lbase) & 4u) != 0u;
+            if (!row.odd) {
+                const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0);
+                c0 = v.x;
+                c1 = v.y;
+            } else {
+                // Cells x - 1, x; lane 63 also x + 1, x + 2 (its right
+                // neighbour's pair).
+                const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0 - 4u);
+                c0 = v.x;
+                c1 = v.y;
+                if (lane == 63) {
+                    const pair_t e = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0 + 4u);
+                    h = e.x;
+                    h2 = e.y;
+                }
+            }
         } else {
             c0 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o0);
             c1 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o1);
         }
-        if (halo_lane) h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oh);
+        if (MODE != kSweepAnyParity || !row.odd)
+            if (halo_lane) h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oh);
         row.c0 = c0;
         row.c1 = c1;
-        row.l = h;  // the halo value until finish() forms l and r
+        row.l = h;  // the halo values until finish() forms l and r
+        row.r = h2;
     };
     auto finish = [&](Row &row) {
-        const float h = row.l;
-        row.l = from_left(row.c1, h);
-        row.r = from_right(row.c0, h);
+        if (MODE == kSweepAnyParity && row.odd) {
+            // Loaded: c0 = cell x - 1, c1 = cell x; lane 63: l, r = cells
+            // x + 1, x + 2. Lane + 1's pair holds cells x + 1, x + 2.
+            const float xm1 = row.c0, x0 = row.c1;
+            const float x1 = from_right(xm1, row.l);
+            const float x2 = from_right(x0, row.r);
+            row.l = xm1;
+            row.c0 = x0;
+            row.c1 = x1;
+            row.r = x2;
+        } else {
+            const float h = row.l;
+            row.l = from_left(row.c1, h);
+            row.r = from_right(row.c0, h);
+        }
         row.hd0 = row.c1 - row.l;
         row.hd1 = row.r - row.c0;
     };
     auto store = [&](const DstView &d, int64_t y, float v0, float v1) {
         const row_t rp = reinterpret_cast<row_t>(gp(d.p) + (y - rg.oy) * d.stride);
-        if constexpr (PAIRS) {
+        if constexpr (MODE == kSweepPairs) {
             // Outputs are not read again by this launch: stream them.
             __builtin_nontemporal_store(pair_t{v0, v1}, reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so));
+        } else if constexpr (MODE == kSweepAnyParity) {
+            if (((reinterpret_cast<uintptr_t>(rp) + sbase) & 4u) == 0u) {
+                __builtin_nontemporal_store(pair_t{v0, v1}, reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so));
+            } else {
+                // Pairs (x - 1, x) from the left lane's second value; the
+                // wave's first and last cells go alone.
+                const float pv = from_left(v1, 0.0f);
+                if (lane != 0)
+                    __builtin_nontemporal_store(pair_t{pv, v0},
+                                                reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so - 4u));
+                else
+                    *reinterpret_cast<STRATA_GLOBAL float *>(rp + so) = v0;
+                if (lane == 63) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so + 4u) = v1;
+            }
         } else {
             if (in0) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so) = v0;
             if (in1) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so + 4u) = v1;
@@ -472,9 +539,9 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
                        xb + kSweepCols <= rg.ox + rg.w;
     if (pairs)
-        sweep<OP, true>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+        sweep<OP, kSweepPairs>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
     else
-        sweep<OP, false>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+        sweep<OP, kSweepChecked>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
 }
 
 // The output views of one launch, by value in the kernel arguments.
@@ -495,9 +562,9 @@ __global__ __launch_bounds__(kSweepThreads) void horn_surface(HornArgs a, Region
     const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
                        xb + kSweepCols <= rg.ox + rg.w;
     if (pairs)
-        sweep<STRATA_OP_SURFACE, true>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
+        sweep<STRATA_OP_SURFACE, kSweepPairs>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
     else
-        sweep<STRATA_OP_SURFACE, false>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
+        sweep<STRATA_OP_SURFACE, kSweepChecked>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v);
 }
 
 // A fan-out of radius-1 terrain ops: horn_sweep's layout, output o is
@@ -514,9 +581,9 @@ __global__ __launch_bounds__(kSweepThreads) void horn_fan(HornArgs a, FanArgs f,
     const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
                        xb + kSweepCols <= rg.ox + rg.w;
     if (pairs)
-        sweep<kOpHornFan, true>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
+        sweep<kOpHornFan, kSweepPairs>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
     else
-        sweep<kOpHornFan, false>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
+        sweep<kOpHornFan, kSweepChecked>(a, rg, s, d.v[0], d.v[0], xb, y0, y1, threadIdx.x, NoPost(), d.v, &f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1345,14 +1412,18 @@ __device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t
     const int64_t xb = t.rg.ox + (lu % t.blocks_x) * kWaveCols;
     const int64_t y0 = t.rg.oy + (lu / t.blocks_x) * t.rows;
     const int64_t y1 = min(y0 + (int64_t)t.rows, t.rg.oy + t.rg.h);
-    const bool pairs = t.aligned && xb >= t.s.lx0 && xb + kWaveCols <= t.s.lx1 && xb >= t.rg.ox &&
-                       xb + kWaveCols <= t.rg.ox + t.rg.w;
+    // Units inside the source buffer and the region move pairs: aligned
+    // ones directly, the others row by row at their own parity.
+    const bool inside = xb >= t.s.lx0 && xb + kWaveCols <= t.s.lx1 && xb >= t.rg.ox &&
+                        xb + kWaveCols <= t.rg.ox + t.rg.w;
     const ChainPost post(&op);
     const uint32_t lane = threadIdx.x & 63;
-    if (pairs)
-        sweep<OP, true>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
+    if (inside && t.aligned)
+        sweep<OP, kSweepPairs>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
+    else if (inside)
+        sweep<OP, kSweepAnyParity>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
     else
-        sweep<OP, false>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
+        sweep<OP, kSweepChecked>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
 }
 
 template <int KIND, int R>
