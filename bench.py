@@ -511,7 +511,7 @@ def pmc_traffic(args, N, log):
                 for row in csv.DictReader(f):
                     name = row.get("Kernel_Name", "")
                     v = float(row["Counter_Value"])
-                    if "horn_sweep<2>" in name:
+                    if "horn_sweep<2, false>" in name or "horn_sweep<2>" in name:  # the aligned Slope sweep
                         slope_vals.append(v)
                     elif "copyBuffer" in name:
                         copy_vals.append(v)

@@ -520,7 +520,11 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     }
 }
 
-template <int OP>
+// UNALIGNED: the operands do not allow pair accesses at even offsets, so
+// blocks inside the buffer and the region take kSweepAnyParity. A separate
+// instantiation: the aligned kernel keeps only the paths it runs (with the
+// third path beside them, the aligned sweep ran 2% slower).
+template <int OP, bool UNALIGNED>
 __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region rg, SrcView s, DstView d0,
                                                             DstView d1)
 {
@@ -536,12 +540,18 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
-    const bool pairs = a.aligned && xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox &&
-                       xb + kSweepCols <= rg.ox + rg.w;
-    if (pairs)
-        sweep<OP, kSweepPairs>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
-    else
-        sweep<OP, kSweepChecked>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+    const bool inside = xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
+    if constexpr (UNALIGNED) {
+        if (inside)
+            sweep<OP, kSweepAnyParity>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+    } else {
+        if (inside && a.aligned)
+            sweep<OP, kSweepPairs>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+    }
 }
 
 // The output views of one launch, by value in the kernel arguments.
@@ -1630,7 +1640,10 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
         e0.p += sb * a.rows * e0.stride;
         e1.p += sb * a.rows * e1.stride;
         const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
-        hipLaunchKernelGGL(horn_sweep<OP>, grid, dim3(kSweepThreads), 0, st, a, sub, s, e0, e1);
+        if (aligned)
+            hipLaunchKernelGGL((horn_sweep<OP, false>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0, e1);
+        else
+            hipLaunchKernelGGL((horn_sweep<OP, true>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0, e1);
     }
     return hipGetLastError();
 }
