@@ -650,9 +650,11 @@ constexpr int focal_lds_floats()
 // One kTileX x kTileY output tile at (tx0, ty0) of the region.
 // For terrain.Surface through the fit (P.op STRATA_OP_SURFACE), dset holds
 // one view per wanted product, in output order.
-// UNROLL: stage and run the column pass with compile-time trip counts (all
-// loads in flight at once). The batch's kernel, which inlines every kind,
-// keeps the rolled loops: unrolled, it needs several times the registers.
+// UNROLL (a kernel per op, not the batch's kernel, which inlines every
+// kind): stage cell by cell and run the column pass with compile-time trip
+// counts, several LDS reads in flight together. Inlined beside the other
+// kinds inside the batch's tile loop, that form needs several times the
+// registers; the batch stages row by row and keeps the column loop rolled.
 template <int KIND, int R, class TAPS, bool UNROLL = true>
 __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps, const Region &rg,
                                             const SrcView &s, const DstView &d0, const DstView &d1, int64_t tx0,
@@ -666,19 +668,17 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
     float(*colv)[CY][LW] = reinterpret_cast<float(*)[CY][LW]>(lds + LH * LW);
 
     const int tid = threadIdx.x;
-    // Staging: every load of the thread is issued before any LDS write
-    // (the trip count is a compile-time constant), and a tile whose staged
-    // window lies inside the buffer takes unchecked loads.
-    constexpr int kStage = (LH * LW + kTileThreads - 1) / kTileThreads;
-    if constexpr (!UNROLL) {
-        __syncthreads();  // the LDS may still be read by the previous tile
-        for (int i = tid; i < LH * LW; i += kTileThreads) {
-            const int row = i / LW, col = i - row * LW;
-            lds[i] = load_cell(s, tx0 - R + col, ty0 - R + row);
-        }
-    } else {
-    float sv[kStage];
-    {
+    // Staging. Both forms issue every load of the thread before any LDS
+    // write. UNROLL (a kernel per op): cells in thread order, unchecked
+    // loads when the staged window lies inside the buffer. Otherwise (the
+    // batch's kernel) row by row: wave w takes staged rows w, w + 4, ...,
+    // lane l columns l and 64 + l (LW <= 80), with uniform row addresses
+    // and no per-cell address arithmetic for the compiler to hoist out of
+    // the batch's tile loop; rows and columns outside the buffer are
+    // clamped into it and read as zeros.
+    if constexpr (UNROLL) {
+        constexpr int kStage = (LH * LW + kTileThreads - 1) / kTileThreads;
+        float sv[kStage];
         const int64_t bx = tx0 - R, by = ty0 - R;
         if (bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1) {
             const STRATA_GLOBAL float *base = gp(s.p) + (by - s.sy) * s.stride + (bx - s.sx);
@@ -696,13 +696,42 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
                 sv[k] = load_cell(s, bx + col, by + row);
             }
         }
-    }
-    __syncthreads();  // the LDS may still be read by the previous tile of a batch
+        __syncthreads();  // the LDS may still be read by the previous tile
 #pragma unroll
-    for (int k = 0; k < kStage; ++k) {
-        const int i = tid + k * kTileThreads;
-        if (i < LH * LW) lds[i] = sv[k];
-    }
+        for (int k = 0; k < kStage; ++k) {
+            const int i = tid + k * kTileThreads;
+            if (i < LH * LW) lds[i] = sv[k];
+        }
+    } else {
+        constexpr int kStageRows = (LH + 3) / 4;
+        const int sw = tid >> 6, sl = tid & 63;
+        const int64_t bx = tx0 - R, by = ty0 - R;
+        const bool cols_in = bx >= s.lx0 && bx + LW <= s.lx1;  // uniform
+        const int64_t ca = min(max(bx + sl, s.lx0), s.lx1 - 1), cb = min(max(bx + 64 + sl, s.lx0), s.lx1 - 1);
+        const bool in_a = cols_in || ca == bx + sl;
+        const bool in_b = sl + 64 < LW && (cols_in || cb == bx + 64 + sl);
+        const int64_t oa = ca - s.sx, ob = cb - s.sx;
+        float va[kStageRows], vb[kStageRows];
+#pragma unroll
+        for (int k = 0; k < kStageRows; ++k) {
+            const int row = sw + 4 * k;
+            const int64_t yy = by + row;
+            const bool rok = row < LH && yy >= s.ly0 && yy < s.ly1;  // uniform
+            const STRATA_GLOBAL float *rp = gp(s.p) + (min(max(yy, s.ly0), s.ly1 - 1) - s.sy) * s.stride;
+            va[k] = rp[oa];
+            vb[k] = rp[ob];
+            va[k] = (rok && in_a) ? va[k] : 0.0f;
+            vb[k] = (rok && in_b) ? vb[k] : 0.0f;
+        }
+        __syncthreads();  // the LDS may still be read by the previous tile of a batch
+#pragma unroll
+        for (int k = 0; k < kStageRows; ++k) {
+            const int row = sw + 4 * k;
+            if (row < LH) {
+                tile[row][sl] = va[k];
+                if (sl + 64 < LW) tile[row][64 + sl] = vb[k];
+            }
+        }
     }
     if constexpr (KIND == FK_CORRELATE && R > 2) {
         constexpr int WP = weight_pitch<R>();
