@@ -1393,6 +1393,104 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
     validity_strip<R>(rg, src, has_src, dst, wx, y0, y1);
 }
 
+// validity_strip's rule for a narrow region, whose row (its output words
+// and four helper lanes) fits in G <= 32 lanes: the wave's lanes form
+// 64 / G groups of G lanes, group g taking the region's rows
+// [y0 + kVgRows g, y0 + kVgRows (g + 1)) below y1, its lane l standing for
+// validity_strip's lane l of the region's first word group (cell word
+// xc + 32 l with xc = rg.ox - 64). Lanes past the last group only compute.
+// Row and word offsets differ between groups, so they are per lane; every
+// load is clamped into the source rows as on validity_strip's checked path,
+// and a row's first and last output words update the row's own bits with
+// atomics (a narrow output's rows share words).
+template <int R>
+__device__ __forceinline__ void validity_groups(const Region &rg, const MaskView &src, int has_src,
+                                                const MaskView &dst, int64_t y0, int64_t y1, int G)
+{
+    constexpr int NL = kVgRows + 2 * R;
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / G, l = lane - g * G;
+    const int64_t ys = y0 + (int64_t)g * kVgRows;                  // the group's first row
+    const int64_t ye = g < 64 / G ? min(ys + kVgRows, y1) : ys;     // and its end
+    const int64_t xc = rg.ox - 64;
+    const uint32_t *sbits = reinterpret_cast<const uint32_t *>(src.bits);
+    uint32_t *dbits = reinterpret_cast<uint32_t *>(dst.bits);
+
+    const int32_t u0 = clamp32(R - xc, -4096, 4096), u1 = clamp32(rg.W - R - xc, -4096, 4096);
+    const uint32_t imask = bit_range(u0 - 32 * l, u1 - 32 * l);
+
+    uint32_t A[NL];
+    if (has_src) {
+        const int64_t p0 = src.off + (ys - R - src.y0) * src.stride + (xc - src.x0);  // bit of cell (xc, ys - R)
+        const int64_t safe = (src.off + (src.ly0 - src.y0) * src.stride + (src.lx0 - src.x0)) >> 5;
+        const int64_t dlo = src.lx0 - xc, dhi = src.lx1 - 1 - xc;
+        int64_t p = p0;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const int64_t yy = ys - R + j;
+            const int64_t base = p >> 5;  // the word of the group's lane 0
+            const int64_t lo = ((p + dlo) >> 5) - base, hi = ((p + dhi) >> 5) - base;
+            const bool ok = yy >= src.ly0 && yy < src.ly1 && lo <= G - 1 && hi >= 0;
+            const int32_t lo32 = ok ? clamp32(lo, 0, G - 1) : 0, hi32 = ok ? clamp32(hi, 0, G - 1) : 0;
+            const int32_t lc = min(max(l, lo32), hi32);
+            const STRATA_GLOBAL uint32_t *rowp = gp(sbits) + (ok ? base + lc : safe);
+            const uint32_t w = *rowp;
+            A[j] = (ok && lc == l) ? w : 0u;
+            p += src.stride;
+        }
+        // Each row's word aligned to the cell grid (its shift recomputed
+        // from the first row's bit rather than kept per row).
+        const uint32_t p32 = (uint32_t)p0, st32 = (uint32_t)src.stride;
+#pragma unroll
+        for (int j = 0; j < NL; ++j)
+            A[j] = __builtin_amdgcn_alignbit(lane_up(A[j]), A[j], (p32 + (uint32_t)j * st32) & 31u);
+    }
+
+    int64_t ob = dst.off + (ys - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, ys)
+#pragma unroll
+    for (int i = 0; i < kVgRows; ++i) {
+        const int64_t y = ys + i;
+        const bool row_in = y >= R && y < rg.H - R;
+        uint32_t v;
+        if (has_src) {
+            uint32_t a = A[i];
+#pragma unroll
+            for (int j = 1; j <= 2 * R; ++j) a &= A[i + j];
+            const uint32_t lo = lane_down(a), hi = lane_up(a);
+            uint32_t e = a;
+#pragma unroll
+            for (int c = 1; c <= R; ++c) {
+                e &= __builtin_amdgcn_alignbit(hi, a, (uint32_t)c);        // cell + c
+                e &= __builtin_amdgcn_alignbit(a, lo, (uint32_t)(32 - c));  // cell - c
+            }
+            v = row_in ? e : 0u;
+        } else {
+            v = row_in ? ~0u : 0u;
+        }
+        v &= imask;
+        const uint32_t s = (uint32_t)(ob & 31);
+        const int64_t wb = (ob >> 5) - 2;  // the output word of the group's lane 0
+        const uint32_t prev = lane_down(v);
+        const uint32_t out = s == 0 ? v : __builtin_amdgcn_alignbit(v, prev, 32u - s);
+        const int64_t oe = ob + rg.w;  // bit after the row
+        const int32_t last = clamp32(((oe - 1) >> 5) - wb, -1, 64);  // lane of the row's last word
+        const uint32_t e = (uint32_t)(oe & 31);
+        if (y < ye && l >= 2 && l <= last) {
+            uint32_t own = ~0u;
+            if (l == 2) own &= ~0u << s;                  // the row's first word
+            if (e != 0 && l == last) own &= (1u << e) - 1u;  // the row's last word
+            uint32_t *word = dbits + wb + l;
+            if (own == ~0u) {
+                *gp(word) = out;
+            } else {
+                atomicAnd(word, ~own | (out & own));
+                atomicOr(word, out & own);
+            }
+        }
+        ob += dst.stride;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // batch_kernel: many tiles and mixed ops, persistent launches over work lists
 // ---------------------------------------------------------------------------
@@ -1497,6 +1595,17 @@ __device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &
 // One validity wave of a task per unit.
 __device__ __forceinline__ void batch_validity(const BatchOp &op, const BatchTask &t, int64_t w)
 {
+    if (t.vgroup) {
+        // 64 / vgroup row blocks of kVgRows rows per unit.
+        const int G = (int)t.vgroup;
+        const int64_t y0 = t.rg.oy + w * (int64_t)(kVgRows * (64 / G)), y1 = t.rg.oy + t.rg.h;
+        switch (op.radius) {
+        case 1: validity_groups<1>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
+        case 2: validity_groups<2>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
+        default: validity_groups<3>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
+        }
+        return;
+    }
     const int64_t wx = w % t.vwaves_x, strip = w / t.vwaves_x;
     const int64_t y0 = t.rg.oy + strip * kVsRows;
     const int64_t y1 = min(y0 + (int64_t)kVsRows, t.rg.oy + t.rg.h);
@@ -1901,6 +2010,11 @@ void batch_units(const BatchOp &op, BatchTask &t, int rows)
         t.blocks_x = (uint32_t)cdiv(t.rg.w, kTileX);
     }
     t.vwaves_x = (uint32_t)cdiv(t.rg.w / 32 + 2, kVsOut);
+    // A narrow region's rows (output words plus four helper lanes) fit in
+    // a group of lanes: several row blocks share a wave.
+    const int64_t g = cdiv(t.rg.w, 32) + 5;
+    t.vgroup = g <= 32 ? (uint32_t)g : 0u;
+    if (t.vgroup) t.vwaves_x = 1;
 }
 
 hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side, hipEvent_t *ev)

@@ -200,6 +200,10 @@ struct BatchOp {
 // of focal tiles) and, when its output has a mask, the validity list (units
 // of one validity wave). [u0, u1) are its units in the list the copy belongs
 // to.
+// Rows per row group of a narrow region's validity units (fewer than a
+// strip's sixteen, so the wave kernel keeps its registers).
+constexpr int kVgRows = 8;
+
 struct BatchTask {
     Region rg;
     SrcView s;
@@ -212,7 +216,7 @@ struct BatchTask {
     uint32_t blocks_x;  // data units across
     uint32_t rows;      // rows per data unit
     uint32_t vwaves_x;  // validity waves across
-    uint32_t pad;
+    uint32_t vgroup;    // lanes per row group of a narrow region's validity units (0: one row per wave)
     int64_t u0, u1;
 };
 
