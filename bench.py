@@ -754,6 +754,10 @@ def cpu_baseline(args, dem, out, slope, N, log):
     from oracle import oracle
     from strata_amd.raster import Float32Raster
 
+    # First, while no oracle threads are still spinning after a parallel run
+    # (they would take the cores the launching thread needs).
+    plain = plain_4096(slope, log)
+
     rows = min(args.cpu_rows, N)
     n = rows * N
     words = (n + 63) // 64
@@ -784,7 +788,59 @@ def cpu_baseline(args, dem, out, slope, N, log):
     cpu = {"value": round(rate, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
            "sample": f"oracle (scalar C restatement) Slope on rows 0-{rows - 1} of the same DEM ({rows}x{N}), "
                      f"{reps} passes, {threads} threads"}
+    cpu["plain_4096"] = plain
     return cpu, parity
+
+
+def plain_4096(slope, log):
+    """The reference's CPU-runnable configuration: plain terrain.Slope
+    (CellSize 30, degrees) on a 4096x4096 fBm DEM with 1% of cells NoData
+    i.i.d. The oracle runs it on one thread, as the plain path does; the
+    device runs the same call (strata_process); the two outputs, data and
+    validity, are compared bit for bit over the whole raster."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle
+    from strata_amd import synth, terrain
+    from strata_amd.raster import Float32Raster
+
+    n = 4096
+    dev = torch.device("cuda:0")
+    data = synth.fbm_dem(n, n, hurst=0.8, seed=3, lo=0.0, hi=3000.0, device=dev)
+    valid = synth.random_nodata(n * n, 0.01, seed=4, device=dev)
+    words = valid.numel()
+    dem = Float32Raster(data, n, n, n, valid, 0)
+    out = Float32Raster(torch.empty(n * n, dtype=torch.float32, device=dev), n, n, n,
+                        torch.zeros(words, dtype=torch.int64, device=dev), 0)
+    opts = terrain.SlopeOptions(CellSize=30)
+    for _ in range(3):
+        terrain.Slope(out, dem, opts)
+    torch.cuda.synchronize()
+    reps = 50
+    t = time.perf_counter()
+    for _ in range(reps):
+        terrain.Slope(out, dem, opts)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t) / reps
+
+    src = Float32Raster(data.cpu().numpy().copy(), n, n, n, valid.cpu().numpy().view(np.uint64).copy(), 0)
+    dst = Float32Raster(np.zeros(n * n, np.float32), n, n, n, np.zeros(words, np.uint64), 0)
+    t = time.perf_counter()
+    oracle.process(slope, [dst], [src], threads=1)
+    cpu_s = time.perf_counter() - t
+
+    got = out.Data.cpu().numpy()
+    same = (got.view(np.uint32) == dst.Data.view(np.uint32)) | (np.isnan(got) & np.isnan(dst.Data))
+    gv = out.Valid.cpu().numpy().view(np.uint64)
+    res = {"workload": "plain terrain.Slope, 4096x4096 fBm DEM (Hurst 0.8, 0-3000 m), 1% NoData i.i.d., CellSize 30",
+           "cpu_1_thread_s": round(cpu_s, 4), "cpu_1_thread_gcells_s": round(n * n / cpu_s / 1e9, 4),
+           "gpu_ms": round(gpu_s * 1e3, 4), "gpu_gcells_s": round(n * n / gpu_s / 1e9, 2),
+           "gpu_over_cpu": round(cpu_s / gpu_s, 1),
+           "identical": bool(same.all() and np.array_equal(gv, dst.Valid))}
+    log(f"bench: plain Slope 4096^2: CPU 1 thread {cpu_s:.3f} s, GPU {gpu_s * 1e3:.3f} ms, identical "
+        f"{res['identical']}")
+    return res
 
 
 if __name__ == "__main__":

@@ -20,6 +20,8 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <condition_variable>
+#include <thread>
 #include <vector>
 
 #include "host.hpp"
@@ -768,52 +770,165 @@ namespace {
 // Raw files (engine/raw.go)
 // ---------------------------------------------------------------------------
 
+// Host threads for raw file IO: a pool created on first use and kept for
+// the process (never destroyed, so no thread is joined at exit). run(n, f)
+// calls f(0..n-1) on the pool and the calling thread and returns when all
+// are done; calls from several threads take turns.
+class IoPool {
+  public:
+    static IoPool &get()
+    {
+        static IoPool *p = new IoPool();
+        return *p;
+    }
+    void run(int n, const std::function<void(int)> &f)
+    {
+        std::lock_guard<std::mutex> turn(run_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            n_ = n;
+            next_ = 0;
+            left_ = n;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return left_ == 0; });
+        job_ = nullptr;
+    }
+    int threads() const { return (int)th_.size() + 1; }
+
+  private:
+    IoPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int n = (int)std::min(7u, hw > 1 ? hw - 1 : 1u);
+        for (int i = 0; i < n; ++i) th_.emplace_back([this] { loop(); });
+    }
+    // Claims parts of the current job until none is left.
+    void work()
+    {
+        for (;;) {
+            int k;
+            const std::function<void(int)> *f;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!job_ || next_ >= n_) return;
+                k = next_++;
+                f = job_;
+            }
+            (*f)(k);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    void loop()
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> th_;
+    const std::function<void(int)> *job_ = nullptr;
+    int n_ = 0, next_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+};
+
+// Rows [r0, r1) of a W-wide raw file at (x, y) to or from buf (row stride
+// `stride`): one pread/pwrite per row, or one for all of them when they are
+// contiguous both in the file and in buf. Returns -1, or the first row (of
+// the file) that failed, with *eof telling a short read from an errno.
+template <bool WRITE>
+int64_t raw_rows(const strata_raw &r, int64_t W, char *buf, int64_t w, int64_t stride, int64_t x, int64_t y,
+                 int64_t r0, int64_t r1, bool *eof, int *err)
+{
+    const bool contiguous = x == 0 && w == W && stride == W;
+    const int64_t step = contiguous ? r1 - r0 : 1;
+    for (int64_t row = r0; row < r1; row += step) {
+        const int64_t nrows = std::min(step, r1 - row);
+        char *p = buf + 4 * row * stride;
+        const size_t total = (size_t)(4 * w * nrows);
+        size_t done = 0;
+        off_t off = (off_t)(r.offset + 4 * ((y + row) * W + x));
+        while (done < total) {
+            const ssize_t n = WRITE ? pwrite(r.fd, p + done, total - done, off) : pread(r.fd, p + done, total - done, off);
+            if (n < 0 && errno == EINTR) continue;
+            if (n <= 0) {
+                *eof = n == 0;
+                *err = n == 0 ? 0 : errno;
+                return y + row + (int64_t)(done / (size_t)(4 * w));
+            }
+            done += (size_t)n;
+            off += n;
+        }
+    }
+    return -1;
+}
+
+// Rows of a tile split into parts of at least 2 MiB for the IO pool; the
+// lowest failing row is the one reported, as a row-by-row loop reports it.
+// Rows of a part past a failure in another part may still have moved.
+template <bool WRITE>
+int64_t raw_io(const strata_raw &r, int64_t W, char *buf, int64_t w, int64_t h, int64_t stride, int64_t x,
+               int64_t y, bool *eof, int *err)
+{
+    const int64_t bytes = 4 * w * h;
+    IoPool &pool = IoPool::get();
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)pool.threads(), h, bytes >> 21}));
+    if (parts == 1) return raw_rows<WRITE>(r, W, buf, w, stride, x, y, 0, h, eof, err);
+    std::vector<int64_t> bad(parts, -1);
+    std::vector<char> beof(parts, 0);
+    std::vector<int> berr(parts, 0);
+    pool.run(parts, [&](int k) {
+        bool e = false;
+        int en = 0;
+        bad[k] = raw_rows<WRITE>(r, W, buf, w, stride, x, y, h * k / parts, h * (k + 1) / parts, &e, &en);
+        beof[k] = e;
+        berr[k] = en;
+    });
+    for (int k = 0; k < parts; ++k)
+        if (bad[k] >= 0) {
+            *eof = beof[k] != 0;
+            *err = berr[k];
+            return bad[k];
+        }
+    return -1;
+}
+
 // Reads rows y..y+h of columns x..x+w of a W-wide raw file into buf (rows
 // `stride` floats apart). Errors as the reference's wrap of a raw source's
 // error (internal/exec/chunked.go:379-393, engine/raw.go:86-92).
 int raw_read(const strata_raw &r, int64_t W, float *buf, int64_t w, int64_t h, int64_t stride, int64_t x, int64_t y,
              const char *what)
 {
-    for (int64_t row = 0; row < h; ++row) {
-        char *p = (char *)(buf + row * stride);
-        size_t left = (size_t)w * 4;
-        off_t off = (off_t)(r.offset + 4 * ((y + row) * W + x));
-        while (left > 0) {
-            const ssize_t n = pread(r.fd, p, left, off);
-            if (n < 0 && errno == EINTR) continue;
-            if (n <= 0)
-                return fail(STRATA_EIO, "engine: reading %s at (%lld, %lld): engine: raw source: reading row %lld: %s",
-                            what, (long long)x, (long long)y, (long long)(y + row),
-                            n == 0 ? "unexpected EOF" : strerror(errno));
-            p += n;
-            left -= (size_t)n;
-            off += n;
-        }
-    }
-    return STRATA_OK;
+    bool eof = false;
+    int err = 0;
+    const int64_t row = raw_io<false>(r, W, (char *)buf, w, h, stride, x, y, &eof, &err);
+    if (row < 0) return STRATA_OK;
+    return fail(STRATA_EIO, "engine: reading %s at (%lld, %lld): engine: raw source: reading row %lld: %s", what,
+                (long long)x, (long long)y, (long long)row, eof ? "unexpected EOF" : strerror(err));
 }
 
 // Writes rows of buf into the W-wide raw file at (x, y).
 int raw_write(const strata_raw &r, int64_t W, const float *buf, int64_t w, int64_t h, int64_t stride, int64_t x,
               int64_t y, int o)
 {
-    for (int64_t row = 0; row < h; ++row) {
-        const char *p = (const char *)(buf + row * stride);
-        size_t left = (size_t)w * 4;
-        off_t off = (off_t)(r.offset + 4 * ((y + row) * W + x));
-        while (left > 0) {
-            const ssize_t n = pwrite(r.fd, p, left, off);
-            if (n < 0 && errno == EINTR) continue;
-            if (n <= 0)
-                return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): engine: raw sink: writing row %lld: %s",
-                            o, (long long)x, (long long)y, (long long)(y + row),
-                            n == 0 ? "short write" : strerror(errno));
-            p += n;
-            left -= (size_t)n;
-            off += n;
-        }
-    }
-    return STRATA_OK;
+    bool eof = false;
+    int err = 0;
+    const int64_t row = raw_io<true>(r, W, (char *)buf, w, h, stride, x, y, &eof, &err);
+    if (row < 0) return STRATA_OK;
+    return fail(STRATA_EIO, "engine: writing dst[%d] at (%lld, %lld): engine: raw sink: writing row %lld: %s", o,
+                (long long)x, (long long)y, (long long)row, eof ? "short write" : strerror(err));
 }
 
 int check_raw(const strata_raw *r, int32_t masked, const char *name, int i)

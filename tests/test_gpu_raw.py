@@ -164,3 +164,40 @@ def test_short_file_writes_a_prefix(gpu, tmp_path):
     assert "reading src[0]" in str(err) and "reading row 30: unexpected EOF" in str(err)
     got = np.fromfile(opath, dtype="<f4").reshape(H, W)
     assert np.all(got[:20] != 0) and np.all(got[20:] == 0)  # tiles 0 and 1 written, the rest untouched
+
+
+def test_large_tiles_split_across_io_threads(gpu, tmp_path):
+    """Tiles of 8 MiB are read and written in parts by the library's IO
+    threads (full-width rows: one pread/pwrite per part); the file is the
+    one the row-by-row reference writes."""
+    W, H = 2048, 1500
+    ipath, opath, epath = (str(tmp_path / n) for n in ("in.raw", "out.raw", "want.raw"))
+    dem_file(ipath, W, H, seed=6)
+    opts = RawOptions(Fill=FILL, HasFill=True)
+    k = terrain.slopeOp(terrain.SlopeOptions(CellSize=30))
+    want = expected_file(k, host_read(ipath, W, H, opts), W, H, opts, epath)
+    fi, fo = OpenRawFile(ipath, os.O_RDONLY), CreateRawFile(opath, 4 * W * H)
+    err = terrain.SlopeChunked(None, RawSink(fo, W, H, opts), RawSource(fi, W, H, opts),
+                               terrain.SlopeOptions(CellSize=30), Options(TileHeight=1024))
+    fi.Close()
+    fo.Close()
+    assert err is None
+    assert same_bits(np.fromfile(opath, dtype="<f4"), want)
+
+
+def test_large_tile_short_file_names_first_missing_row(gpu, tmp_path):
+    """A file that ends inside a tile read in parts: the error names the
+    first row that is missing, as a row-by-row read would."""
+    W, H = 2048, 1200
+    ipath, opath = str(tmp_path / "in.raw"), str(tmp_path / "out.raw")
+    dem_file(ipath, W, H, seed=7)
+    with open(ipath, "r+b") as fh:
+        fh.truncate(4 * W * 700 + 12)  # row 700 only partly there
+    opts = RawOptions(Fill=FILL, HasFill=True)
+    fi, fo = OpenRawFile(ipath, os.O_RDONLY), CreateRawFile(opath, 4 * W * H)
+    err = terrain.SlopeChunked(None, RawSink(fo, W, H, opts), RawSource(fi, W, H, opts),
+                               terrain.SlopeOptions(CellSize=30), Options(TileHeight=1024))
+    fi.Close()
+    fo.Close()
+    assert isinstance(err, SourceError)
+    assert "reading src[0]" in str(err) and "reading row 700: unexpected EOF" in str(err), str(err)
