@@ -123,14 +123,15 @@ __device__ __forceinline__ float sqrt_fast(float x)
     return rup > 0.0f ? yup : r;
 }
 
-// Correctly rounded n / d for |n| in [2^-30, 2^30] and d in [1, 2^30]: the
-// compiler's refinement sequence (reciprocal refined by one Newton step, two
-// residual corrections) without the range scaling and fix-up, which never
-// apply there.
+// Correctly rounded n / d for |n| and d in [2^-30, 2^30]: the compiler's
+// refinement sequence (reciprocal refined by one Newton step, two residual
+// corrections) without its range scaling and fix-up, which apply only to
+// operands or quotients near overflow, underflow or the denormals (the
+// quotient here lies in [2^-60, 2^60]).
 __device__ __forceinline__ unsigned div_fast_ok(float n, float d)
 {
     const float an = u2f(f2u(n) & 0x7fffffffu);
-    return (an >= 0x1p-30f) & (an <= 0x1p30f) & (d >= 1.0f) & (d <= 0x1p30f);
+    return (an >= 0x1p-30f) & (an <= 0x1p30f) & (d >= 0x1p-30f) & (d <= 0x1p30f);
 }
 __device__ __forceinline__ float div_fast(float n, float d)
 {
@@ -201,6 +202,90 @@ __device__ __forceinline__ void slope_pair(float gx0, float gy0, float gx1, floa
     }
 }
 
+// n / d for a pair, with the fast quotient when the whole wave is in its
+// range.
+__device__ __forceinline__ void div_pair(float n0, float d0, float n1, float d1, float &q0, float &q1)
+{
+    if (wave_all(div_fast_ok(n0, d0) & div_fast_ok(n1, d1))) {
+        q0 = div_fast(n0, d0);
+        q1 = div_fast(n1, d1);
+    } else {
+        q0 = n0 / d0;
+        q1 = n1 / d1;
+    }
+}
+
+// atan32 of a pair (x >= 0, +Inf or NaN): the same reduction, its quotient
+// through div_pair. Below tan(pi/8) the reduction divides by 1, which is
+// exact, so t = x there and that lane lends the quotient a 1 / 1.
+__device__ __forceinline__ void atan32_pair(float x0, float x1, float &a0, float &a1)
+{
+    auto reduce = [](float x, float &num, float &den, float &y0, bool &small) {
+        num = x;
+        den = 1.0f;
+        y0 = 0.0f;
+        const bool mid = x > kTanPi8, hi = x > kTan3Pi8;
+        if (mid) {
+            num = x - 1.0f;
+            den = x + 1.0f;
+            y0 = u2f(kPi4Bits);
+        }
+        if (hi) {
+            num = -1.0f;
+            den = x;
+            y0 = u2f(kPi2Bits);
+        }
+        small = !mid;  // x <= tan(pi/8), or NaN
+    };
+    float n0, d0, y00, n1, d1, y01;
+    bool s0, s1;
+    reduce(x0, n0, d0, y00, s0);
+    reduce(x1, n1, d1, y01, s1);
+    float t0, t1;
+    div_pair(s0 ? 1.0f : n0, s0 ? 1.0f : d0, s1 ? 1.0f : n1, s1 ? 1.0f : d1, t0, t1);
+    t0 = s0 ? n0 : t0;  // x / 1 == x (x is a quotient, so a NaN is already quiet)
+    t1 = s1 ? n1 : t1;
+    auto poly = [](float t, float y0) {
+        const float z = t * t;
+        float p = kAtanC4 * z - kAtanC3;
+        p = p * z + kAtanC2;
+        p = p * z - kAtanC1;
+        return y0 + ((p * z) * t + t);
+    };
+    a0 = poly(t0, y00);
+    a1 = poly(t1, y01);
+}
+
+// aspect_of for a pair: Atan2F32's quotient |y| / |x| and atan32's through
+// the pair forms, the rest as aspect_of (internal/stencil/aspect.go:36-48,
+// 94-123).
+__device__ __forceinline__ void aspect_pair(float gx0, float gy0, float gx1, float gy1, float flat, bool trig,
+                                            float &o0, float &o1)
+{
+    const float y0 = trig ? gy0 : 0.0f - gx0, x0 = trig ? 0.0f - gx0 : gy0;
+    const float y1 = trig ? gy1 : 0.0f - gx1, x1 = trig ? 0.0f - gx1 : gy1;
+    const float ax0 = u2f(f2u(x0) & 0x7fffffffu), ay0 = u2f(f2u(y0) & 0x7fffffffu);
+    const float ax1 = u2f(f2u(x1) & 0x7fffffffu), ay1 = u2f(f2u(y1) & 0x7fffffffu);
+    float q0, q1, a0, a1;
+    div_pair(ay0, ax0, ay1, ax1, q0, q1);
+    atan32_pair(q0, q1, a0, a1);
+    auto finish = [&](float a, float x, float y, float ax, float ay) {
+        if (ay == 0.0f && ax == 0.0f) a = 0.0f;
+        if (ay == ax && ay > 3.40282346638528859811704183484516925440e+38f) a = u2f(kPi4Bits);
+        if (f2u(x) & 0x80000000u) a = u2f(kPiBits) - a;
+        a = u2f(f2u(a) | (f2u(y) & 0x80000000u));
+        float d = a * u2f(kRadToDegBits);
+        float off = 0.0f;
+        if (d < 0.0f) off = 360.0f;
+        d += off;
+        if (d >= 360.0f) d = 0.0f;
+        if (y == 0.0f && x == 0.0f) d = flat;
+        return d;
+    };
+    o0 = finish(a0, x0, y0, ax0, ay0);
+    o1 = finish(a1, x1, y1, ax1, ay1);
+}
+
 // Hillshade of a pair (shade_of with the fast root and quotient when the
 // whole wave is in their ranges).
 __device__ __forceinline__ void shade_pair(float gx0, float gy0, float gx1, float gy1, float c, float bx, float by,
@@ -268,8 +353,7 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
         } else if constexpr (OP == STRATA_OP_SLOPE) {
             slope_pair(gx0, gy0, gx1, gy1, a.k[2], (a.flags & 1) != 0, o00, o01);
         } else if constexpr (OP == STRATA_OP_ASPECT) {
-            o00 = aspect_of(gx0, gy0, a.k[2], (a.flags & 2) != 0);
-            o01 = aspect_of(gx1, gy1, a.k[2], (a.flags & 2) != 0);
+            aspect_pair(gx0, gy0, gx1, gy1, a.k[2], (a.flags & 2) != 0, o00, o01);
         } else {
             shade_pair(gx0, gy0, gx1, gy1, a.k[2], a.k[3], a.k[4], o00, o01);
         }
@@ -461,9 +545,11 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                         slope_pair(gx0, gy0, gx1, gy1, a.k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
                         put(o++, v0, v1);
                     }
-                    if (f & STRATA_SURF_ASPECT)
-                        put(o++, aspect_of(gx0, gy0, a.k[3], (f & STRATA_FLAG_TRIG) != 0),
-                            aspect_of(gx1, gy1, a.k[3], (f & STRATA_FLAG_TRIG) != 0));
+                    if (f & STRATA_SURF_ASPECT) {
+                        float v0, v1;
+                        aspect_pair(gx0, gy0, gx1, gy1, a.k[3], (f & STRATA_FLAG_TRIG) != 0, v0, v1);
+                        put(o++, v0, v1);
+                    }
                     if (f & STRATA_SURF_HILLSHADE) {
                         float v0, v1;
                         shade_pair(gx0, gy0, gx1, gy1, a.k[4], a.k[5], a.k[6], v0, v1);
@@ -494,8 +580,7 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                                 slope_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
                                 break;
                             case STRATA_OP_ASPECT:
-                                v0 = aspect_of(gx0, gy0, k[2], (f & STRATA_FLAG_TRIG) != 0);
-                                v1 = aspect_of(gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                                aspect_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0, v0, v1);
                                 break;
                             case STRATA_OP_HILLSHADE:
                                 shade_pair(gx0, gy0, gx1, gy1, k[2], k[3], k[4], v0, v1);
@@ -2103,9 +2188,9 @@ __global__ void selfcheck_kernel(int kind, uint64_t lo, uint64_t n, unsigned lon
                 differ = f2u(atan32_small(sqrt_fast(x))) != f2u(atan32(__builtin_sqrtf(x)));
             }
         } else {
-            // |n| in [2^-30, 2^30) with random sign and mantissa, d in [1, 2^30).
+            // |n| and d in [2^-30, 2^30), n with a random sign, random mantissas.
             const uint32_t h1 = mix32(lo * 0x9e3779b97f4a7c15ull + 2 * i), h2 = mix32(lo + 2 * i + 1);
-            const uint32_t en = 127 - 30 + (h1 >> 26) % 60, ed = 127 + (h2 >> 27) % 30;
+            const uint32_t en = 127 - 30 + (h1 >> 26) % 60, ed = 127 - 30 + (h2 >> 26) % 60;
             const float num = u2f((h1 & 0x80000000u) | (en << 23) | (h1 & 0x7fffffu));
             const float den = u2f((ed << 23) | (h2 & 0x7fffffu));
             if (div_fast_ok(num, den)) {
