@@ -579,8 +579,9 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                             case STRATA_OP_SLOPE:
                                 slope_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
                                 break;
-                            case STRATA_OP_ASPECT:
-                                aspect_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0, v0, v1);
+                            case STRATA_OP_ASPECT:  // the short form: this loop's code stays small
+                                v0 = aspect_of(gx0, gy0, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                                v1 = aspect_of(gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0);
                                 break;
                             case STRATA_OP_HILLSHADE:
                                 shade_pair(gx0, gy0, gx1, gy1, k[2], k[3], k[4], v0, v1);
