@@ -1145,18 +1145,60 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     if (tid == 0) nonfinite = 0;
     for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
     __syncthreads();
-    bool any = false;
-#pragma unroll 4
-    for (int i = tid; i < LH * PITCH; i += 256) {
-        const int row = i / PITCH, col = i - row * PITCH;
-        float v = col < LW ? load_cell(s, tx0 - R + col, ty0 - R + row) : 0.0f;
-        if (!__builtin_isfinite(v)) {
-            any = true;
-            v = 0.0f;
+    // Staging in pairs of columns: every load of the thread is issued before
+    // any split (the trip count is a compile-time constant), a tile whose
+    // staged window lies inside the buffer takes unchecked loads, and each
+    // pair's high and low parts go to LDS as one 32-bit store each.
+    {
+        constexpr int PPR = PITCH / 2;  // pairs per staged row
+        constexpr int NP = LH * PPR;
+        constexpr int KST = (NP + 255) / 256;
+        const int64_t bx = tx0 - R, by = ty0 - R;
+        const bool inside = bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1;
+        float va[KST], vb[KST];
+        if (inside) {
+            const STRATA_GLOBAL float *base = gp(s.p) + (by - s.sy) * s.stride + (bx - s.sx);
+#pragma unroll
+            for (int k = 0; k < KST; ++k) {
+                const int pi = min(tid + k * 256, NP - 1);
+                const int row = pi / PPR, col = 2 * (pi - row * PPR);
+                const STRATA_GLOBAL float *rp = base + (int64_t)row * s.stride;
+                va[k] = col < LW ? rp[col] : 0.0f;
+                vb[k] = col + 1 < LW ? rp[col + 1] : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KST; ++k) {
+                const int pi = min(tid + k * 256, NP - 1);
+                const int row = pi / PPR, col = 2 * (pi - row * PPR);
+                va[k] = col < LW ? load_cell(s, bx + col, by + row) : 0.0f;
+                vb[k] = col + 1 < LW ? load_cell(s, bx + col + 1, by + row) : 0.0f;
+            }
         }
-        split_bf16(v, hi[i], lo[i]);
+        bool any = false;
+        uint32_t *hi32 = reinterpret_cast<uint32_t *>(hi), *lo32 = reinterpret_cast<uint32_t *>(lo);
+#pragma unroll
+        for (int k = 0; k < KST; ++k) {
+            const int pi = tid + k * 256;
+            if (pi < NP) {
+                float a = va[k], b = vb[k];
+                if (!__builtin_isfinite(a)) {
+                    any = true;
+                    a = 0.0f;
+                }
+                if (!__builtin_isfinite(b)) {
+                    any = true;
+                    b = 0.0f;
+                }
+                __bf16 ha, la, hb, lb;
+                split_bf16(a, ha, la);
+                split_bf16(b, hb, lb);
+                hi32[pi] = (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+                lo32[pi] = (uint32_t)__builtin_bit_cast(uint16_t, la) | ((uint32_t)__builtin_bit_cast(uint16_t, lb) << 16);
+            }
+        }
+        if (any) nonfinite = 1;
     }
-    if (any) nonfinite = 1;
     __syncthreads();
 
     const int n = lane & 15, kg = lane >> 4;
