@@ -1130,8 +1130,32 @@ __device__ __forceinline__ float correlate_cell(const float *taps, const SrcView
     return acc;
 }
 
+// The Toeplitz operands of every weight row, split: entry (i, lane) holds
+// the lane's B operand of T_i (k = 8(lane/16) .. +7, n = lane % 16) as its
+// bf16 high and low parts. Built once per call by N waves; every workgroup
+// of correlate_mfma then reads it (from L2) instead of building it.
 template <int R>
-__global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg, SrcView s, DstView d0)
+__global__ __launch_bounds__(64) void mfma_toeplitz(const Params P, bf16x8 *tab)
+{
+    constexpr int N = 2 * R + 1;
+    const int i = blockIdx.x, lane = threadIdx.x, n = lane & 15, kg = lane >> 4;
+    bf16x8 bh, bl;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int j = 8 * kg + e - n;
+        const float w = (j >= 0 && j < N) ? P.taps[i * N + j] : 0.0f;
+        __bf16 h, l;
+        split_bf16(w, h, l);
+        bh[e] = h;
+        bl[e] = l;
+    }
+    gp(tab)[(i * 64 + lane) * 2] = bh;
+    gp(tab)[(i * 64 + lane) * 2 + 1] = bl;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg, SrcView s, DstView d0,
+                                                      const bf16x8 *__restrict__ tab)
 {
     constexpr int N = 2 * R + 1;
     constexpr int LH = kMfmaTile + 2 * R, LW = kMfmaTile + 2 * R, PITCH = mfma_pitch<R>();
@@ -1210,16 +1234,7 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     // registers at once.
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
-        bf16x8 bh, bl;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int j = 8 * kg + e - n;
-            const float w = (j >= 0 && j < N) ? wts[i * N + j] : 0.0f;
-            __bf16 h, l;
-            split_bf16(w, h, l);
-            bh[e] = h;
-            bl[e] = l;
-        }
+        const bf16x8 bh = gp(tab)[(i * 64 + lane) * 2], bl = gp(tab)[(i * 64 + lane) * 2 + 1];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int at = (16 * b + n + i) * PITCH + c0 + 8 * kg;  // A: row m = n, columns 8kg..8kg+7
@@ -1988,6 +2003,11 @@ static hipError_t run_focal(const Params &P, const Region &rg, const SrcView &s,
 template <int R>
 static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s, const DstView *d, hipStream_t st)
 {
+    constexpr int N = 2 * R + 1;
+    bf16x8 *tab = nullptr;
+    hipError_t e = hipMallocAsync((void **)&tab, sizeof(bf16x8) * 2 * 64 * N, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mfma_toeplitz<R>, dim3(N), dim3(64), 0, st, P, tab);
     const int64_t gx = cdiv(rg.w, kMfmaTile);
     for (int64_t yb = 0; yb < cdiv(rg.h, kMfmaTile); yb += kMaxGridY) {
         Region sub = rg;
@@ -1996,9 +2016,11 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
         DstView e0 = d[0];
         e0.p += yb * kMfmaTile * e0.stride;
         hipLaunchKernelGGL(correlate_mfma<R>, dim3((unsigned)gx, (unsigned)cdiv(sub.h, kMfmaTile)), dim3(256), 0, st,
-                           P, sub, s, e0);
+                           P, sub, s, e0, tab);
     }
-    return hipGetLastError();
+    e = hipGetLastError();
+    const hipError_t f = hipFreeAsync(tab, st);
+    return e != hipSuccess ? e : f;
 }
 
 static hipError_t run_mfma(const Params &P, const Region &rg, const SrcView &s, const DstView *d, hipStream_t st)
