@@ -1881,7 +1881,7 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
 
 // Launch-shape tuning (never changes a result).
-static Tuning g_tuning{32, 4096, {0, 0, 0, 0, 0, 0}};
+static Tuning g_tuning{16, 4096, {0, 0, 0, 0, 0, 0}};
 static int sweep_rows() { return g_tuning.sweep_rows; }
 
 Tuning get_tuning() { return g_tuning; }
