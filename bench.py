@@ -278,7 +278,16 @@ def main() -> int:
                      "traffic": traffic["bytes_per_launch"] if traffic and "bytes_per_launch" in traffic else None,
                      "traffic_detail": traffic,
                      "kernel": "horn_sweep<SLOPE> (data kernel)", "bytes_per_launch": data_bytes,
-                     "kernel_ms": round(kern_ms, 4), "launches": timing.data_launches},
+                     "kernel_ms": round(kern_ms, 4), "launches": timing.data_launches,
+                     # engine.Stats of one op (the reference's byte accounting, in closed
+                     # form) beside the bytes the device moved (counters, data kernel).
+                     "stats_check": {"ideal_bytes": st.Ideal, "engine_total_bytes": st.Total(),
+                                     "engine_kernel_read": st.KernelRead,
+                                     "engine_kernel_written": st.KernelWritten,
+                                     "device_hbm_bytes": (traffic["bytes_per_launch"]
+                                                          if traffic and "bytes_per_launch" in traffic else None),
+                                     "device_over_ideal": (round(traffic["bytes_per_launch"] / st.Ideal, 4)
+                                                           if traffic and "bytes_per_launch" in traffic else None)}},
         "op_roofline": {"bytes_per_step": op_bytes, "frac": round(op_bytes / (el / args.steps) / 1e9 /
                                                                    HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
