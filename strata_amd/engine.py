@@ -816,6 +816,10 @@ class TeeSink(RasterSink):
                 raise EnginePanic(f"engine: tee sink's sink {i} is nil")
             if t.Size() != (width, height):
                 raise EnginePanic(f"engine: tee sink is {width}×{height}, its sink {i} is {t.Size()}")
+            # The library hands the tee's one sink the tee's validity as is.
+            if t.Masked() != bool(masked):
+                raise EnginePanic(f"engine: tee sink is {'' if masked else 'not '}Masked but its sink {i} is "
+                                  f"{'' if t.Masked() else 'not '}Masked")
         self.w, self.h, self.masked = width, height, bool(masked)
         self.sinks = list(sinks)
         self.what = what

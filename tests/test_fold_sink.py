@@ -96,6 +96,10 @@ def test_tee_checks_and_targets():
         TeeSink(5, 3, True, [s])
     with pytest.raises(EnginePanic, match="sink 1 is nil"):
         TeeSink(4, 3, True, [s, None])
+    with pytest.raises(EnginePanic, match="tee sink is Masked but its sink 0 is not Masked"):
+        TeeSink(4, 3, True, [MemorySink(host(4, 3, masked=False))])
+    with pytest.raises(EnginePanic, match="tee sink is not Masked but its sink 0 is Masked"):
+        TeeSink(4, 3, False, [s])
     only = TeeSink(4, 3, True)
     assert _sink_target(only) is None  # statistics alone write nothing
     one = TeeSink(4, 3, True, [s])
