@@ -276,3 +276,27 @@ def test_known_answer_plane_on_gpu(gpu):
         g = data_grid(dst)
         assert np.isnan(g[0]).all() and np.isnan(g[-1]).all() and np.isnan(g[:, 0]).all()
         assert np.abs(g[1:-1, 1:-1] - want).max() <= tol
+
+
+@pytest.mark.parametrize("sx,sw_extra,dx,dw_extra", [(1, 0, 0, 0), (0, 1, 1, 0), (1, 1, 1, 1), (3, 2, 2, 3),
+                                                     (0, 0, 1, 1)])
+def test_unaligned_blocks_every_parity(gpu, sx, sw_extra, dx, dw_extra):
+    """Whole 512-column blocks over operands that allow no pair accesses at
+    even offsets: the source and output windows start at odd or even
+    columns of parents with odd or even strides, so a row's parity can
+    alternate from one row to the next. Every radius-1 terrain op, against
+    the oracle, whole parent buffers compared."""
+    W, H = 1100, 40
+    ks = [(n, k) for n, k in ALL if k.radius == 1 and not (k.flags & 0x100) and n.split("-")[0].rstrip("0123456789")
+          in ("gradient", "slope", "aspect", "hillshade", "curv")]
+    assert len(ks) >= 6
+    for name, k in ks:
+        rng = np.random.default_rng(hash((name, sx, dx)) & 0xFFFF)
+        sc = Scene()
+        sc.add_parent("src", rng, W + 8 + sw_extra, H + 2, masked=True, valid_base=5)
+        sc.add_window("s", "src", sx, 1, W, H)
+        for o in range(k.nout):
+            sc.add_parent(f"dst{o}", rng, W + 4 + dw_extra, H + 1, masked=True, valid_base=17 + 8 * o)
+            sc.add_window(f"d{o}", f"dst{o}", dx, 1, W, H)
+        got, want = run_case(gpu, k, sc)
+        assert_same_parents(got, want, f"{name} sx {sx} dx {dx} strides +{sw_extra}/+{dw_extra}")
