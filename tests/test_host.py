@@ -225,6 +225,28 @@ def test_option_panics(fn, prefix):
     assert str(e.value).startswith(prefix)
 
 
+@pytest.mark.parametrize("fn,prefix", [
+    (lambda: terrain.slopeOp(terrain.SlopeOptions(CellSize=1, FitRadius=-1)), "terrain: FitRadius"),
+    (lambda: terrain.slopeOp(terrain.SlopeOptions(CellSize=1, FitRadius=9)), "terrain: FitRadius"),
+    (lambda: terrain.newSurface((False, False, True, False, False), terrain.SurfaceOptions(CellSize=1, FitRadius=-1)),
+     "terrain: FitRadius"),
+    (lambda: terrain.newSurface((False, False, True, False, False), terrain.SurfaceOptions(CellSize=1, FitRadius=9)),
+     "terrain: FitRadius"),
+    (lambda: terrain.curvatureOp(terrain.CurvatureOptions(FitRadius=2)), "terrain: CellSize"),
+    (lambda: terrain.slopeOp(terrain.SlopeOptions(CellSize=1, FitRadius=2, Units=7)), "terrain: unknown SlopeUnits"),
+    (lambda: terrain.hillshadeOp(terrain.HillshadeOptions(CellSize=1, FitRadius=2, Altitude=95)), "terrain: Altitude"),
+    (lambda: terrain.slopeOp(terrain.SlopeOptions(CellSize=1e30, ZFactor=1e-30, FitRadius=2)),
+     "terrain: the quadratic fit's factors"),
+])
+def test_fit_panics(fn, prefix):
+    """terrain/fit_test.go:250-264: out-of-range fit radii (ops and Surface),
+    a fit without a cell size, bad units or altitude with a fit, and fit
+    factors that underflow float32."""
+    with pytest.raises(terrain.TerrainPanic) as e:
+        fn()
+    assert str(e.value).startswith(prefix)
+
+
 def host(w, h, masked=False, stride=None):
     stride = stride or w
     n = (h - 1) * stride + w

@@ -3,8 +3,9 @@ against the closed forms the reference's own fit tests hold
 (terrain/fit_test.go:35-132): on integer planes the fitted gradient is the
 plane's and every curvature is +0; on a quadratic surface, which least
 squares fits exactly, the gradient and the three curvatures are the
-surface's own, to the float32 rounding bound those tests derive. CPU only."""
-import math
+surface's own, to the float32 rounding bound those tests derive; a fitted
+output's validity is the radius-r box rule (terrain/fit_test.go:230-248).
+CPU only."""
 
 import numpy as np
 import pytest
@@ -132,3 +133,28 @@ def test_fit_quadratic_fails_a_dropped_term():
             v = -(q * q * rr - 2 * p * q * ss + p * p * tt) / g ** 1.5
             misses += abs(float(got[row, col]) - v) > 2e-3 * abs(v) + 1e-6
     assert misses > 0
+
+
+@pytest.mark.parametrize("r", [1, 3, 8])
+def test_fit_validity_is_the_radius_r_box(r):
+    """terrain/fit_test.go:230-248: a fitted output is valid exactly where
+    its (2r+1)² window lies inside the raster and holds only valid inputs
+    (the reference compares with a radius-r box op's validity; here the box
+    rule itself, brute force)."""
+    from helpers import hazard_values
+    from strata_amd.raster import Float32Raster, pack_bits, valid_grid
+
+    rng = np.random.default_rng(83 + r)
+    w, h = 70, 2 * r + 11
+    bits = rng.random(w * h) >= 0.02
+    dem = Float32Raster(hazard_values(rng, w * h), w, h, w, pack_bits(bits), 0)
+    sv = bits.reshape(h, w)
+    want = np.zeros((h, w), bool)
+    for y in range(r, h - r):
+        for x in range(r, w - r):
+            want[y, x] = sv[y - r:y + r + 1, x - r:x + r + 1].all()
+    for k in (terrain.slopeOp(terrain.SlopeOptions(CellSize=1, FitRadius=r)),
+              terrain.curvatureOp(terrain.CurvatureOptions(CellSize=1, FitRadius=r))):
+        got = NewFloat32Like(dem)
+        oracle.process(k, [got], [dem])
+        assert (valid_grid(got) == want).all(), k.name
