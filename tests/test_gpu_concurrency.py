@@ -1,8 +1,9 @@
 """Concurrent calls from many host threads. The reference's entry points may
 be called from many goroutines at once, and tiling and workers never change
 a result (internal/exec/kernel.go:14-19, engine/doc.go:32-41). Here eight
-threads make a shuffled mix of plain and Chunked calls over device and host
-operands, and each result must equal the same call made alone. ctypes
+threads, half of them on streams of their own, make a shuffled mix of plain
+and Chunked calls over device and host operands, and each result must equal
+the same call made alone. ctypes
 releases the GIL around each library call, so the calls overlap inside the
 library."""
 import threading
@@ -82,14 +83,24 @@ def test_threads_match_calls_made_alone():
 
     failures, errors = [], []
 
-    def worker(seed):
+    def calls(seed):
         order = np.random.default_rng(seed)
+        for _ in range(3):
+            for i in order.permutation(len(items)):
+                j, w = items[i]
+                if not same(run(js[j][1], js[j][2], data, valid, w), alone[(j, w)]):
+                    failures.append((seed, js[j][0], w))
+
+    def worker(seed):
+        # Odd threads share the default stream; even ones each launch on a
+        # stream of their own (default priority), where device operands are
+        # also made and read back.
         try:
-            for _ in range(3):
-                for i in order.permutation(len(items)):
-                    j, w = items[i]
-                    if not same(run(js[j][1], js[j][2], data, valid, w), alone[(j, w)]):
-                        failures.append((seed, js[j][0], w))
+            if seed % 2:
+                calls(seed)
+            else:
+                with torch.cuda.stream(torch.cuda.Stream(device=torch.device("cuda:0"))):
+                    calls(seed)
         except Exception as e:  # reported below; a thread's exception does not fail the test by itself
             errors.append((seed, repr(e)))
 
