@@ -15,7 +15,8 @@ Zevenbergen–Thorne's sums have a fixed order, so these hold bit for bit
 
 Each side runs through its own path (plain, Tiled or Chunked over memory
 sources and sinks) on a DEM of a few million cells; inputs are made and
-results compared on the host.
+results compared on the host. Scaling and negation also run over the
+20000² DEM, compared on the device.
 """
 import numpy as np
 import pytest
@@ -183,3 +184,36 @@ def test_negation_is_negative_zfactor():
         got = run(kk, nout, -data, valid, (i + 1) % 3)
         for o in range(nout):
             require(got[o], want[o], same_value=name == "gradient")
+
+
+def test_scaling_and_negation_at_20000():
+    """Power-of-two scaling against ZFactor and negation against a negative
+    ZFactor, for Slope and Hillshade over the 20000² fBm DEM with 0.5%
+    NoData blobs (the headline workload), compared bit for bit on the
+    device."""
+    from strata_amd import synth
+
+    N = 20000
+    dev = torch.device("cuda:0")
+    data = synth.fbm_dem(N, N, hurst=0.8, seed=1, lo=0.0, hi=3000.0, device=dev)
+    valid, _ = synth.blob_nodata(N, N, frac=0.005, rmin=5, rmax=50, seed=2, device=dev)
+
+    def go(k, d):
+        out = Float32Raster(torch.empty(N * N, dtype=torch.float32, device=dev), N, N, N, torch.zeros_like(valid), 0)
+        assert ProcessChunked(None, [MemorySink(out)], [MemorySource(Float32Raster(d, N, N, N, valid, 0))], k,
+                              Options(TileHeight=256)) is None
+        return out
+
+    bits = ((valid.view(-1, 1) >> torch.arange(64, device=dev)) & 1).reshape(-1)[:N * N].bool()
+    for name, mk in (("slope", lambda z: T.slopeOp(T.SlopeOptions(CellSize=30, ZFactor=z))),
+                     ("hillshade", lambda z: T.hillshadeOp(T.HillshadeOptions(CellSize=30, ZFactor=z)))):
+        base = go(mk(1.0), data)
+        ok = base.Valid
+        want = base.Data[bits]
+        for d, z in ((data * 0.125, 8.0), (data * 4.0, 0.25), (-data, -1.0)):
+            got = go(mk(z), d)
+            assert torch.equal(got.Valid, ok), (name, z)
+            assert torch.equal(got.Data[bits].view(torch.int32), want.view(torch.int32)), (name, z)
+            del got
+        del base, want
+        torch.cuda.empty_cache()
