@@ -1376,7 +1376,6 @@ __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, Src
 // Words only partly owned by the region's row are updated with atomics on
 // the row's own bits; all reads stay inside the source buffer's rows.
 constexpr int kVsOut = 60;   // output words per wave and row
-constexpr int kVsRows = 16;  // rows per wave
 
 __device__ __forceinline__ uint32_t lane_up(uint32_t v)  // v of lane+1 (lane 63: 0)
 {
