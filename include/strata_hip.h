@@ -352,6 +352,21 @@ int strata_reduce(int what, const strata_raster *src, const strata_options *opts
 int strata_reduce_chunked(int what, const strata_source *src, const strata_options *opts, strata_fold *out,
                           strata_stats *stats);
 
+/*
+ * strata_reduce over a device raster without allocating anything. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The fold's scratch (the exponent bins,
+ * the running total and one partial per workgroup, under 128 KiB) is a
+ * static device array that the library holds on each GPU it runs on until
+ * the process ends. Results, Stats and cancellation are strata_reduce's.
+ * src must be in device memory and hold at most 2^40 cells (STRATA_EINVAL
+ * otherwise). Calls that overlap on one GPU share its scratch and race:
+ * the caller serializes them.
+ */
+int strata_reduce_noalloc(int what, const strata_raster *src, const strata_options *opts, strata_fold *out,
+                          strata_stats *stats);
+
 /* ------------------------------------------------------------------ */
 /* Launch tuning (no counterpart in the reference): shapes only, never  */
 /* results                                                              */

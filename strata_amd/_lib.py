@@ -266,6 +266,7 @@ EXPORTS = (
     "strata_plan_stats_fanout",
     "strata_reduce",
     "strata_reduce_chunked",
+    "strata_reduce_noalloc",
     "strata_get_tuning",
     "strata_set_tuning",
     "strata_selfcheck",
@@ -332,6 +333,8 @@ def lib() -> ctypes.CDLL:
     L.strata_reduce.restype = c_int
     L.strata_reduce_chunked.argtypes = [c_int, POINTER(Source), POINTER(Options), POINTER(Fold), POINTER(Stats)]
     L.strata_reduce_chunked.restype = c_int
+    L.strata_reduce_noalloc.argtypes = [c_int, POINTER(Raster), POINTER(Options), POINTER(Fold), POINTER(Stats)]
+    L.strata_reduce_noalloc.restype = c_int
     L.strata_get_tuning.argtypes = [POINTER(Tuning)]
     L.strata_get_tuning.restype = c_int
     L.strata_set_tuning.argtypes = [POINTER(Tuning)]

@@ -623,6 +623,39 @@ int scratch_pool(hipMemPool_t *out)
     return STRATA_OK;
 }
 
+// Device memory of strata_reduce_noalloc: the fold's bins, its running
+// total and one partial per workgroup of the fold launch. Each GPU the
+// library runs on holds its own copy from the moment the code object is
+// loaded until the process ends. Every call on one GPU uses that GPU's copy.
+constexpr size_t kNoAllocScratchBytes = 128 << 10;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The largest raster strata_reduce_noalloc folds: the int64 bins hold the
+// exact sums of at most 2^40 values.
+constexpr int64_t kNoAllocMaxCells = (int64_t)1 << 40;
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current. In host code the array's name is only a stand-in for
+// the symbol; device work must be given this address.
+int scratch_address(void **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(STRATA_EDEVICE, "strata: device %d past the scratch table", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "locating the no-allocation scratch");
+        if (!p) return fail(STRATA_EDEVICE, "strata: the no-allocation scratch has no device address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return STRATA_OK;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -635,6 +668,9 @@ int strata_init(int device)
 {
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     HIP_TRY(hipFree(nullptr), "device init");
+    void *p = nullptr;
+    const int rc = scratch_address(&p);
+    if (rc) return rc;
     g_err.clear();
     return STRATA_OK;
 }
@@ -1552,6 +1588,54 @@ int strata_reduce(int what, const strata_raster *src, const strata_options *opts
         strata_stats s{};
         const Plan p = new_plan(src->width, src->height, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
         for (int64_t b = 0; b < p.bands; ++b) fold_band_stats(s, plan_band(p, b));
+        add_stats(stats, s, 1, 0);
+    }
+    g_err.clear();
+    return STRATA_OK;
+}
+
+// strata_reduce without any allocation: the fold runs in the library's
+// static device scratch (the bins, the running total and the workgroups'
+// partials) instead of memory taken from the stream-ordered allocator.
+// Device rasters only. Calls that overlap on one GPU share that scratch
+// and race; the fold's first kernel clears the bins and the total, so
+// nothing of an earlier call carries over.
+int strata_reduce_noalloc(int what, const strata_raster *src, const strata_options *opts, strata_fold *out,
+                          strata_stats *stats)
+{
+    static_assert(sizeof(FoldBins) + 256 + sizeof(FoldPart) * kFoldMaxBlocks <= kNoAllocScratchBytes,
+                  "the fold's scratch outgrew the static array");
+    int rc = check_reduce(what, opts);
+    if (rc) return rc;
+    if (!src) return fail(STRATA_EINVAL, "engine: reducer takes 1 inputs, got 0");
+    if ((rc = check_raster("src", 0, *src))) return rc;
+    if (src->mem != STRATA_MEM_DEVICE)
+        return fail(STRATA_EINVAL,
+                    "strata: strata_reduce_noalloc reads device rasters only (a host raster needs staging memory)");
+    if (src->width > kNoAllocMaxCells / src->height)
+        return fail(STRATA_EINVAL, "strata: a %lld×%lld raster exceeds strata_reduce_noalloc's limit of 2^40 cells",
+                    (long long)src->width, (long long)src->height);
+    if (out) memset(out, 0, sizeof *out);
+    if (cancelled(opts)) return STRATA_CANCELLED;
+    void *base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
+    char *dev = (char *)base;
+    FoldBins *bins = (FoldBins *)dev;
+    FoldPart *total = (FoldPart *)(dev + sizeof(FoldBins));
+    FoldPart *parts = (FoldPart *)(dev + sizeof(FoldBins) + 256);
+    HIP_TRY(launch_fold(fold_args(*src, what, src->valid != nullptr), bins, parts, total, true, st),
+            "launching the fold");
+    FoldBins b;
+    FoldPart t;
+    HIP_TRY(hipMemcpyAsync(&b, bins, sizeof b, hipMemcpyDeviceToHost, st), "reading the bins");
+    HIP_TRY(hipMemcpyAsync(&t, total, sizeof t, hipMemcpyDeviceToHost, st), "reading the fold");
+    HIP_TRY(hipStreamSynchronize(st), "synchronizing");
+    if (out) fold_result(b, t, what, out);
+    if (stats) {
+        strata_stats s{};
+        const Plan p = new_plan(src->width, src->height, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
+        for (int64_t i = 0; i < p.bands; ++i) fold_band_stats(s, plan_band(p, i));
         add_stats(stats, s, 1, 0);
     }
     g_err.clear();
