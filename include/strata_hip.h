@@ -379,7 +379,11 @@ typedef struct strata_tuning {
      * (strata_trim). Smaller steps hold less memory but launch more,
      * shorter passes (1..4096, default 4096). */
     int32_t fold_stripe_mb;
-    int32_t reserved[6];
+    /* Cells per unit, in 1024s, of a call that can be cancelled: the work
+     * claimed between two polls of the cancel flag. 0: the library's
+     * choice, an eighth of the raster and at least 4 Mi cells. 0..2^20. */
+    int32_t unit_kcells;
+    int32_t reserved[5];
 } strata_tuning;
 
 int strata_get_tuning(strata_tuning *t);

@@ -324,7 +324,10 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
     strata_batch *b = new strata_batch;
     b->tasks.assign(tasks, tasks + ntasks);
     // Own copies of every table, so the caller's taps need not outlive
-    // this call.
+    // this call. A task has at most three tables; the outer vector holds
+    // them all without growing, and an inner vector's data stays put when
+    // the outer one moves it anyway.
+    b->taps.reserve(3 * (size_t)ntasks);
     for (auto &t : b->tasks) {
         auto keep = [&](strata_kernel &k) {
             if (k.ntaps > 0) {
@@ -332,12 +335,9 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
                 k.taps = b->taps.back().data();
             }
         };
-        b->taps.reserve(b->taps.size() + 3);
         keep(t.k);
         for (int j = 0; j < t.npost; ++j) keep(t.post[j]);
     }
-    // Note: taps vectors never reallocate after their kernel points at them
-    // (each is its own vector, and the outer vector only holds them).
     b->op_of.resize(ntasks);
     for (int i = 0; i < ntasks; ++i) {
         const BatchOp op = make_op(b->tasks[i]);

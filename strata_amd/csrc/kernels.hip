@@ -1880,7 +1880,7 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
 
 // Launch-shape tuning (never changes a result).
-static Tuning g_tuning{16, 4096, {0, 0, 0, 0, 0, 0}};
+static Tuning g_tuning{16, 4096, 0, {0, 0, 0, 0, 0}};
 static int sweep_rows() { return g_tuning.sweep_rows; }
 
 Tuning get_tuning() { return g_tuning; }
@@ -1889,6 +1889,7 @@ bool set_tuning(const Tuning &t)
 {
     if (t.sweep_rows < 4 || t.sweep_rows > 4096) return false;
     if (t.fold_stripe_mb < 1 || t.fold_stripe_mb > 4096) return false;
+    if (t.unit_kcells < 0 || t.unit_kcells > (1 << 20)) return false;
     g_tuning = t;
     return true;
 }
@@ -2004,7 +2005,7 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
 {
     constexpr int N = 2 * R + 1;
     bf16x8 *tab = nullptr;
-    hipError_t e = hipMallocAsync((void **)&tab, sizeof(bf16x8) * 2 * 64 * N, st);
+    hipError_t e = scratch_alloc_async((void **)&tab, sizeof(bf16x8) * 2 * 64 * N, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mfma_toeplitz<R>, dim3(N), dim3(64), 0, st, P, tab);
     const int64_t gx = cdiv(rg.w, kMfmaTile);

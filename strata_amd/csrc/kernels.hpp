@@ -75,10 +75,15 @@ struct Params {
 struct Tuning {
     int32_t sweep_rows;      // rows per block of the radius-1 sweep
     int32_t fold_stripe_mb;  // output MiB per step of a resident fold-only pass
-    int32_t reserved[6];
+    int32_t unit_kcells;     // cells per unit of a cancellable call, in 1024s (0: the default)
+    int32_t reserved[5];
 };
 Tuning get_tuning();
 bool set_tuning(const Tuning &t);
+
+// Stream-ordered device memory from the library's pool, which keeps what
+// calls free for later calls (freed with hipFreeAsync on the same stream).
+hipError_t scratch_alloc_async(void **p, size_t bytes, hipStream_t st);
 
 // Data of every cell in the region: the op over interior cells, NaN over
 // the edge ring. nout is 1, 2 (Gradient) or up to 5 (Surface). Returns a

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <string>
 #include <condition_variable>
@@ -26,6 +27,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "plan.hpp"
 #include "strata_hip.h"
 
 using namespace strata;
@@ -49,11 +51,6 @@ int hip_fail(hipError_t e, const char *what)
 {
     return fail(STRATA_EDEVICE, "strata: %s: %s", what, hipGetErrorString(e));
 }
-
-constexpr int64_t kBytesPerCell = 4;
-constexpr int64_t kBandCells = 1 << 16;  // internal/exec/tile.go:27
-
-int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------------------
 // Kernel descriptors
@@ -269,54 +266,6 @@ int check_raster(const char *name, int i, const strata_raster &r)
 
 namespace {
 
-// A host-memory raster mirrored in device memory for the call. Device
-// memory holds the raster's whole span (cells and padding) and the mask
-// words its bits touch; only cells and own bits are copied back.
-struct Staged {
-    strata_raster dev{};
-    void *dbuf = nullptr;
-    uint64_t *dmask = nullptr;
-    int64_t w0 = 0, nwords = 0;
-    ~Staged()
-    {
-        if (dbuf) (void)hipFree(dbuf);
-        if (dmask) (void)hipFree(dmask);
-    }
-};
-
-int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
-{
-    s.dev = h;
-    s.dev.mem = STRATA_MEM_DEVICE;
-    const int64_t n = data_len(h);
-    HIP_TRY(hipMalloc(&s.dbuf, (size_t)n * sizeof(float)), "staging data");
-    s.dev.data = (float *)s.dbuf;
-    if (copy_data)
-        HIP_TRY(hipMemcpyAsync(s.dbuf, h.data, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st), "copy in");
-    if (h.valid) {
-        s.w0 = h.valid_offset >> 6;
-        const int64_t w1 = (h.valid_offset + n - 1) >> 6;
-        s.nwords = w1 - s.w0 + 1;
-        HIP_TRY(hipMalloc((void **)&s.dmask, (size_t)s.nwords * 8), "staging mask");
-        HIP_TRY(hipMemcpyAsync(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, hipMemcpyHostToDevice, st),
-                "copy mask in");
-        s.dev.valid = s.dmask;
-        s.dev.valid_offset = h.valid_offset - 64 * s.w0;
-    }
-    return STRATA_OK;
-}
-
-int stage_out(const strata_raster &h, Staged &s, hipStream_t st)
-{
-    HIP_TRY(hipMemcpy2DAsync(h.data, (size_t)h.stride * 4, s.dbuf, (size_t)h.stride * 4, (size_t)h.width * 4,
-                             (size_t)h.height, hipMemcpyDeviceToHost, st),
-            "copy out");
-    if (h.valid)
-        HIP_TRY(hipMemcpyAsync(h.valid + s.w0, s.dmask, (size_t)s.nwords * 8, hipMemcpyDeviceToHost, st),
-                "copy mask out");
-    return STRATA_OK;
-}
-
 SrcView srcview(const strata_raster &r, int64_t sx, int64_t sy, int64_t W, int64_t H)
 {
     SrcView v;
@@ -344,135 +293,6 @@ MaskView mask_view(const strata_raster &r, int64_t x0, int64_t y0, int64_t W, in
     m.lx1 = std::min<int64_t>(W, x0 + r.width);
     m.ly1 = std::min<int64_t>(H, y0 + r.height);
     return m;
-}
-
-// ---------------------------------------------------------------------------
-// Plan (internal/exec/tile.go:298-359) and Stats (engine/stats.go)
-// ---------------------------------------------------------------------------
-
-struct Plan {
-    int64_t w = 0, h = 0, tileW = 0, tileH = 0, bandRows = 0, tilesX = 0, tileRows = 0, perTile = 0,
-            perLastTile = 0, bands = 0;
-};
-
-Plan new_plan(int64_t w, int64_t h, int64_t tileW, int64_t tileH)
-{
-    Plan p;
-    p.w = w;
-    p.h = h;
-    p.tileW = tileW > 0 ? std::min(tileW, w) : w;
-    p.tileH = tileH > 0 ? std::min(tileH, h) : h;
-    if (w == 0 || h == 0) return p;
-    p.bandRows = std::max<int64_t>(1, kBandCells / p.tileW);
-    p.tilesX = cdiv(w, p.tileW);
-    p.tileRows = cdiv(h, p.tileH);
-    p.perTile = cdiv(p.tileH, p.bandRows);
-    p.perLastTile = cdiv(h - (p.tileRows - 1) * p.tileH, p.bandRows);
-    p.bands = p.tilesX * ((p.tileRows - 1) * p.perTile + p.perLastTile);
-    return p;
-}
-
-struct Rect {
-    int64_t x0, y0, x1, y1;
-};
-
-Rect plan_band(const Plan &p, int64_t i)
-{
-    const int64_t perRow = p.tilesX * p.perTile;
-    int64_t tr = i / perRow, j = i % perRow, per = p.perTile;
-    if (tr >= p.tileRows - 1) {
-        tr = p.tileRows - 1;
-        j = i - (p.tileRows - 1) * perRow;
-        per = p.perLastTile;
-    }
-    const int64_t tx = j / per, b = j % per;
-    Rect r;
-    r.x0 = tx * p.tileW;
-    r.x1 = std::min(r.x0 + p.tileW, p.w);
-    const int64_t ty = tr * p.tileH;
-    r.y0 = ty + b * p.bandRows;
-    r.y1 = std::min(std::min(r.y0 + p.bandRows, ty + p.tileH), p.h);
-    return r;
-}
-
-// One band's counters (internal/exec/halo.go:22-39, 97-105), for the
-// band [x0, x1) x [y0, y1) of a W x H raster: the kernel runs over the
-// cells outside the narrowest edge ring (rmin) and reads its window grown
-// by the radius r.
-void band_stats(strata_stats &s, int r, int rmin, int64_t W, int64_t H, int nin, int nout, const Rect &b)
-{
-    const int64_t cells = (b.x1 - b.x0) * (b.y1 - b.y0);
-    const int64_t ix0 = std::max<int64_t>(b.x0, rmin), ix1 = std::min<int64_t>(b.x1, W - rmin);
-    const int64_t iy0 = std::max<int64_t>(b.y0, rmin), iy1 = std::min<int64_t>(b.y1, H - rmin);
-    const bool interior = ix0 < ix1 && iy0 < iy1;
-    const int64_t inner = interior ? (ix1 - ix0) * (iy1 - iy0) : 0;
-    s.cells += cells;
-    s.bands += 1;
-    s.kernel_written += (cells - inner) * kBytesPerCell * nout;
-    if (interior) {
-        s.kernel_read += (ix1 - ix0 + 2 * r) * (iy1 - iy0 + 2 * r) * kBytesPerCell * nin;
-        s.kernel_written += inner * kBytesPerCell * nout;
-    }
-}
-
-void add_stats(strata_stats *out, const strata_stats &s, int nin, int nout)
-{
-    if (!out) return;
-    out->cells += s.cells;
-    out->ideal += s.cells * kBytesPerCell * (nin + nout);
-    out->tiles += s.tiles;
-    out->bands += s.bands;
-    out->source_read += s.source_read;
-    out->sink_written += s.sink_written;
-    out->kernel_read += s.kernel_read;
-    out->kernel_written += s.kernel_written;
-}
-
-// The chunked tile grid (internal/exec/chunked.go:159-174).
-struct TileGrid {
-    int64_t W, H, tileW, tileH, tilesX, tiles;
-};
-
-TileGrid tile_grid(int64_t W, int64_t H, const strata_options *o)
-{
-    TileGrid g;
-    g.W = W;
-    g.H = H;
-    g.tileW = (o && o->tile_width > 0) ? std::min(o->tile_width, W) : W;
-    g.tileH = (o && o->tile_height > 0) ? std::min(o->tile_height, H) : H;
-    g.tilesX = cdiv(W, g.tileW);
-    g.tiles = g.tilesX * cdiv(H, g.tileH);
-    return g;
-}
-
-Rect tile_rect(const TileGrid &g, int64_t i)
-{
-    Rect t;
-    t.x0 = (i % g.tilesX) * g.tileW;
-    t.y0 = (i / g.tilesX) * g.tileH;
-    t.x1 = std::min(t.x0 + g.tileW, g.W);
-    t.y1 = std::min(t.y0 + g.tileH, g.H);
-    return t;
-}
-
-// One chunked tile's counters (chunked.go:335-374): the source window,
-// the sink tile, and the tile's own plan of bands.
-void tile_stats(strata_stats &s, int r, int rmin, const TileGrid &g, int nin, int nout, const Rect &t)
-{
-    const int64_t rx0 = std::max<int64_t>(0, t.x0 - r), ry0 = std::max<int64_t>(0, t.y0 - r);
-    const int64_t rx1 = std::min<int64_t>(g.W, t.x1 + r), ry1 = std::min<int64_t>(g.H, t.y1 + r);
-    s.tiles += 1;
-    s.source_read += (rx1 - rx0) * (ry1 - ry0) * kBytesPerCell * nin;
-    const Plan p = new_plan(t.x1 - t.x0, t.y1 - t.y0, 0, 0);
-    for (int64_t b = 0; b < p.bands; ++b) {
-        Rect band = plan_band(p, b);
-        band.x0 += t.x0;
-        band.x1 += t.x0;
-        band.y0 += t.y0;
-        band.y1 += t.y0;
-        band_stats(s, r, rmin, g.W, g.H, nin, nout, band);
-    }
-    s.sink_written += (t.x1 - t.x0) * (t.y1 - t.y0) * kBytesPerCell * nout;
 }
 
 // ---------------------------------------------------------------------------
@@ -623,6 +443,63 @@ int scratch_pool(hipMemPool_t *out)
     return STRATA_OK;
 }
 
+// A host-memory raster mirrored in device memory for a call: the
+// raster's whole span (cells and padding) and the mask words its bits
+// touch, in memory from the library's stream-ordered pool (reused from call
+// to call, given back when the call's stream reaches the end of the call).
+struct Staged {
+    strata_raster dev{};
+    void *dbuf = nullptr;
+    uint64_t *dmask = nullptr;
+    int64_t w0 = 0, nwords = 0;
+    hipStream_t st = nullptr;
+    ~Staged()
+    {
+        if (dbuf) (void)hipFreeAsync(dbuf, st);
+        if (dmask) (void)hipFreeAsync(dmask, st);
+    }
+};
+
+int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
+{
+    hipMemPool_t pool = nullptr;
+    int rc = scratch_pool(&pool);
+    if (rc) return rc;
+    s.st = st;
+    s.dev = h;
+    s.dev.mem = STRATA_MEM_DEVICE;
+    const int64_t n = data_len(h);
+    HIP_TRY(hipMallocFromPoolAsync(&s.dbuf, (size_t)n * sizeof(float), pool, st), "staging data");
+    s.dev.data = (float *)s.dbuf;
+    if (copy_data)
+        HIP_TRY(hipMemcpyAsync(s.dbuf, h.data, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st), "copy in");
+    if (h.valid) {
+        s.w0 = h.valid_offset >> 6;
+        const int64_t w1 = (h.valid_offset + n - 1) >> 6;
+        s.nwords = w1 - s.w0 + 1;
+        HIP_TRY(hipMallocFromPoolAsync((void **)&s.dmask, (size_t)s.nwords * 8, pool, st), "staging mask");
+        HIP_TRY(hipMemcpyAsync(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, hipMemcpyHostToDevice, st),
+                "copy mask in");
+        s.dev.valid = s.dmask;
+        s.dev.valid_offset = h.valid_offset - 64 * s.w0;
+    }
+    return STRATA_OK;
+}
+
+// The staged output's cells (rows of width cells, not the padding) and the
+// mask words its bits touch back into the caller's raster. The mask words
+// were copied in whole, so bits of other cells go back as they came.
+int stage_out(const strata_raster &h, Staged &s, hipStream_t st)
+{
+    HIP_TRY(hipMemcpy2DAsync(h.data, (size_t)h.stride * 4, s.dbuf, (size_t)h.stride * 4, (size_t)h.width * 4,
+                             (size_t)h.height, hipMemcpyDeviceToHost, st),
+            "copy out");
+    if (h.valid)
+        HIP_TRY(hipMemcpyAsync(h.valid + s.w0, s.dmask, (size_t)s.nwords * 8, hipMemcpyDeviceToHost, st),
+                "copy mask out");
+    return STRATA_OK;
+}
+
 // Device memory of strata_reduce_noalloc: the fold's bins, its running
 // total and one partial per workgroup of the fold launch. Each GPU the
 // library runs on holds its own copy from the moment the code object is
@@ -633,6 +510,9 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 // The largest raster strata_reduce_noalloc folds: the int64 bins hold the
 // exact sums of at most 2^40 values.
 constexpr int64_t kNoAllocMaxCells = (int64_t)1 << 40;
+
+// Frees the pinned host buffers no call holds (defined with their cache).
+void pinned_trim();
 
 std::mutex g_scratch_mu;
 void *g_scratch_ptr[kMaxDevices] = {};
@@ -681,9 +561,12 @@ const char *strata_version(void) { return "strata-hip 0.1 gfx950"; }
 
 int strata_trim(void)
 {
-    std::lock_guard<std::mutex> lk(g_pool_mu);
-    for (hipMemPool_t p : g_pools)
-        if (p) HIP_TRY(hipMemPoolTrimTo(p, 0), "trimming the scratch pool");
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (hipMemPool_t p : g_pools)
+            if (p) HIP_TRY(hipMemPoolTrimTo(p, 0), "trimming the scratch pool");
+    }
+    pinned_trim();
     g_err.clear();
     return STRATA_OK;
 }
@@ -747,6 +630,7 @@ int strata_get_tuning(strata_tuning *t)
     memset(t, 0, sizeof *t);
     t->sweep_rows = g.sweep_rows;
     t->fold_stripe_mb = g.fold_stripe_mb;
+    t->unit_kcells = g.unit_kcells;
     return STRATA_OK;
 }
 
@@ -756,9 +640,12 @@ int strata_set_tuning(const strata_tuning *t)
     Tuning g = get_tuning();
     g.sweep_rows = t->sweep_rows;
     g.fold_stripe_mb = t->fold_stripe_mb;
+    g.unit_kcells = t->unit_kcells;
     if (!set_tuning(g))
-        return fail(STRATA_EINVAL, "strata: sweep_rows %d outside [4, 4096] or fold_stripe_mb %d outside [1, 4096]",
-                    t->sweep_rows, t->fold_stripe_mb);
+        return fail(STRATA_EINVAL,
+                    "strata: sweep_rows %d outside [4, 4096], fold_stripe_mb %d outside [1, 4096] or unit_kcells %d "
+                    "outside [0, 1048576]",
+                    t->sweep_rows, t->fold_stripe_mb, t->unit_kcells);
     g_err.clear();
     return STRATA_OK;
 }
@@ -995,6 +882,138 @@ int plan_stats(const Job &J, int nsrc, int64_t width, int64_t height, const stra
     return STRATA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Reused host memory and streams of staged and cancellable calls
+// ---------------------------------------------------------------------------
+
+// Pinned host buffers kept for reuse: a call takes the smallest free buffer
+// that holds what it needs, or maps a new one (rounded up to 2 MiB), and
+// gives it back when it ends, so calls of a repeated shape map nothing.
+// strata_trim frees the buffers not in use.
+class PinnedCache {
+  public:
+    static PinnedCache &get()
+    {
+        static PinnedCache *c = new PinnedCache();
+        return *c;
+    }
+    int acquire(size_t bytes, void **out)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end()) {
+                *out = it->second;
+                free_.erase(it);
+                return STRATA_OK;
+            }
+        }
+        const size_t cap = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+        void *p = nullptr;
+        HIP_TRY(hipHostMalloc(&p, cap, hipHostMallocDefault), "pinned buffer");
+        std::lock_guard<std::mutex> lk(mu_);
+        size_[p] = cap;
+        *out = p;
+        return STRATA_OK;
+    }
+    void release(void *p)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(size_[p], p);
+    }
+    void trim()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &e : free_) {
+            (void)hipHostFree(e.second);
+            size_.erase(e.second);
+        }
+        free_.clear();
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> size_;
+};
+
+void pinned_trim() { PinnedCache::get().trim(); }
+
+// Device memory for a call's buffers from the library's stream-ordered pool.
+int pool_alloc(void **p, size_t bytes, hipStream_t st, const char *what)
+{
+    hipMemPool_t pool = nullptr;
+    int rc = scratch_pool(&pool);
+    if (rc) return rc;
+    HIP_TRY(hipMallocFromPoolAsync(p, bytes, pool, st), what);
+    return STRATA_OK;
+}
+
+// Cells per unit of a cancellable call: an eighth of the raster, at least
+// kUnitCells, unless tuned.
+int64_t unit_target(int64_t W, int64_t H)
+{
+    const int64_t tuned = (int64_t)get_tuning().unit_kcells * 1024;
+    if (tuned > 0) return tuned;
+    return std::max<int64_t>(kUnitCells, cdiv(W * H, 8));
+}
+
+// A cancellable call: units of the plan (plan_unit) claimed in order on the
+// call's stream, two in flight. Before claiming unit k the call waits for
+// unit k-2's event, so the device always has the next unit queued while the
+// host checks the flag; the flag is checked before every claim and a
+// claimed unit always finishes, so the finished bands are a prefix of the
+// plan (internal/exec/worker.go:144-181, internal/exec/doc.go:165-183).
+// Adds the finished units' band Stats to s; *any tells whether a unit ran.
+int run_units(const Job &J, const Operands &op, const Plan &p, const strata_options *opts, hipStream_t st,
+              int nsrc, strata_stats &s, bool *any)
+{
+    const int64_t W = op.W, H = op.H;
+    const int64_t target = unit_target(W, H);
+    hipEvent_t done[2] = {nullptr, nullptr};
+    struct Events {
+        hipEvent_t *e;
+        ~Events()
+        {
+            for (int k = 0; k < 2; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } guard{done};
+    for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreateWithFlags(&done[k], hipEventDisableTiming), "unit event");
+    Unit pend[2];
+    bool live[2] = {false, false};
+    auto finish = [&](int k) -> int {
+        if (!live[k]) return STRATA_OK;
+        live[k] = false;
+        HIP_TRY(hipEventSynchronize(done[k]), "waiting for a unit");
+        for (int64_t i = pend[k].b0; i < pend[k].b1; ++i)
+            band_stats(s, J.r, J.rmin, W, H, nsrc, op.nout, plan_band(p, i));
+        return STRATA_OK;
+    };
+    int rc = STRATA_OK, status = STRATA_OK;
+    int64_t n = 0;
+    for (int64_t i = 0; i < p.bands; ++n) {
+        const int k = (int)(n & 1);
+        if ((rc = finish(k))) return rc;
+        if (cancelled(opts)) {
+            status = STRATA_CANCELLED;
+            break;
+        }
+        const Unit u = plan_unit(p, i, target);
+        if ((rc = run_rect(J, op, u.rc, st))) return rc;
+        HIP_TRY(hipEventRecord(done[k], st), "unit event");
+        pend[k] = u;
+        live[k] = true;
+        *any = true;
+        i = u.b1;
+    }
+    // Claimed units always finish, cancelled or not.
+    for (int k = 0; k < 2; ++k)
+        if ((rc = finish(k))) return rc;
+    return status;
+}
+
 int process(const Job &J, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
             const strata_options *opts, strata_stats *stats)
 {
@@ -1003,67 +1022,52 @@ int process(const Job &J, const strata_raster *dst, int ndst, const strata_raste
     if ((rc = ensure_device())) return rc;
     hipStream_t st = opts ? (hipStream_t)opts->stream : nullptr;
     const int64_t W = dst[0].width, H = dst[0].height;
-    const int r = J.r, rmin = J.rmin;
-
-    // Host operands are mirrored on the device for the call.
-    bool any_host = src[0].mem == STRATA_MEM_HOST;
-    for (int o = 0; o < ndst; ++o) any_host = any_host || dst[o].mem == STRATA_MEM_HOST;
-    Staged ssrc, sdst[kMaxDst];
-    const strata_raster *dsrc = &src[0];
-    const strata_raster *ddst[kMaxDst];
-    for (int o = 0; o < ndst; ++o) ddst[o] = &dst[o];
-    if (src[0].mem == STRATA_MEM_HOST) {
-        if ((rc = stage_in(src[0], ssrc, true, st))) return rc;
-        dsrc = &ssrc.dev;
-    }
-    for (int o = 0; o < ndst; ++o)
-        if (dst[o].mem == STRATA_MEM_HOST) {
-            if ((rc = stage_in(dst[o], sdst[o], false, st))) return rc;
-            ddst[o] = &sdst[o].dev;
-        }
-
-    Operands op{dsrc, 0, 0, {}, {}, {}, ndst, W, H};
-    for (int o = 0; o < ndst; ++o) op.dst[o] = ddst[o];
-    strata_stats s{};
     const Plan p = new_plan(W, H, opts ? opts->tile_width : 0, opts ? opts->tile_height : 0);
     const bool can_cancel = opts && opts->cancel;
-    int status = STRATA_OK;
-    if (!can_cancel) {
-        // The plan never changes a result, so the whole raster is one launch.
-        if ((rc = run_rect(J, op, Rect{0, 0, W, H}, st))) return rc;
-        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, r, rmin, W, H, nsrc, ndst, plan_band(p, b));
-    } else {
-        // Units of whole bands, in plan order, within one tile; the check
-        // comes before each claim and a claimed unit always completes.
-        int64_t b = 0;
-        while (b < p.bands) {
-            if (cancelled(opts)) {
-                status = STRATA_CANCELLED;
-                break;
-            }
-            Rect u = plan_band(p, b);
-            strata_stats us{};
-            band_stats(us, r, rmin, W, H, nsrc, ndst, u);
-            int64_t e = b + 1;
-            while (e < p.bands) {
-                const Rect nb = plan_band(p, e);
-                if (nb.x0 != u.x0 || nb.x1 != u.x1 || nb.y0 != u.y1) break;
-                if ((u.x1 - u.x0) * (u.y1 - u.y0) >= kUnitCells) break;
-                band_stats(us, r, rmin, W, H, nsrc, ndst, nb);
-                u.y1 = nb.y1;
-                ++e;
-            }
-            if ((rc = run_rect(J, op, u, st))) return rc;
-            HIP_TRY(hipStreamSynchronize(st), "waiting for a unit");
-            s.cells += us.cells;
-            s.bands += us.bands;
-            s.kernel_read += us.kernel_read;
-            s.kernel_written += us.kernel_written;
-            b = e;
+    // A context done before the call writes nothing (internal/exec/doc.go:
+    // 165-178).
+    if (can_cancel && cancelled(opts)) {
+        g_err.clear();
+        return STRATA_CANCELLED;
+    }
+
+    // Host operands are mirrored in device memory for the call. A host
+    // output's mirror starts as the caller's raster when the call can be
+    // cancelled, so the bands it never claims go back unchanged; otherwise
+    // every cell is computed and the mirror needs no copy in.
+    bool any_host = src[0].mem == STRATA_MEM_HOST;
+    Staged ssrc, sdst[kMaxDst];
+    const strata_raster *dsrc = &src[0];
+    Operands op{dsrc, 0, 0, {}, {}, {}, ndst, W, H};
+    if (src[0].mem == STRATA_MEM_HOST) {
+        if ((rc = stage_in(src[0], ssrc, true, st))) return rc;
+        op.src = &ssrc.dev;
+    }
+    for (int o = 0; o < ndst; ++o) {
+        op.dst[o] = &dst[o];
+        if (dst[o].mem == STRATA_MEM_HOST) {
+            any_host = true;
+            if ((rc = stage_in(dst[o], sdst[o], can_cancel, st))) return rc;
+            op.dst[o] = &sdst[o].dev;
         }
     }
-    for (int o = 0; o < ndst; ++o)
-        if (dst[o].mem == STRATA_MEM_HOST && (rc = stage_out(dst[o], sdst[o], st))) return rc;
+
+    strata_stats s{};
+    int status = STRATA_OK;
+    bool any_unit = false;
+    if (!can_cancel) {
+        // Nothing to poll: the plan never changes a result, so the whole
+        // raster is one launch and its Stats follow in closed form.
+        if ((rc = run_rect(J, op, Rect{0, 0, W, H}, st))) return rc;
+        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, J.r, J.rmin, W, H, nsrc, ndst, plan_band(p, b));
+        any_unit = true;
+    } else {
+        status = run_units(J, op, p, opts, st, nsrc, s, &any_unit);
+        if (status < 0) return status;
+    }
+    if (any_unit)
+        for (int o = 0; o < ndst; ++o)
+            if (dst[o].mem == STRATA_MEM_HOST && (rc = stage_out(dst[o], sdst[o], st))) return rc;
     if (any_host || !(opts && opts->async) || can_cancel) HIP_TRY(hipStreamSynchronize(st), "synchronizing");
     add_stats(stats, s, nsrc, ndst);
     g_err.clear();
@@ -1226,8 +1230,8 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     for (int o = 0; o < ndst; ++o) {
         fold_first[o] = true;
         folds[o].st = st;
-        if (dst[o].fold)
-            HIP_TRY(hipMallocAsync((void **)&folds[o].dev, FoldScratch::bytes(), st), "fold scratch");
+        if (dst[o].fold && (rc = pool_alloc((void **)&folds[o].dev, FoldScratch::bytes(), st, "fold scratch")))
+            return rc;
     }
     // Folds output o's cells of rect rc (raster coordinates), held in d
     // whose cell (0, 0) is raster cell (dx, dy): exec.Reduce over the
@@ -1281,11 +1285,13 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     hipEvent_t done[kSlots] = {nullptr, nullptr};
     auto buf = [&](int slot, int i) -> Buf & { return bufs[slot * nbuf + i]; };
     auto free_bufs = [&]() {
+        // Pinned buffers go back to the cache only once no copy uses them.
+        (void)hipStreamSynchronize(st);
         for (auto &b : bufs) {
-            if (b.hdata) (void)hipHostFree(b.hdata);
-            if (b.hmask) (void)hipHostFree(b.hmask);
-            if (b.ddata) (void)hipFree(b.ddata);
-            if (b.dmask) (void)hipFree(b.dmask);
+            PinnedCache::get().release(b.hdata);
+            PinnedCache::get().release(b.hmask);
+            if (b.ddata) (void)hipFreeAsync(b.ddata, st);
+            if (b.dmask) (void)hipFreeAsync(b.dmask, st);
         }
         for (auto &e : done)
             if (e) (void)hipEventDestroy(e);
@@ -1298,12 +1304,12 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         b.stride = m ? round64(w) : w;
         b.cells = (h - 1) * b.stride + w;
         b.words = m ? cdiv(b.cells, 64) : 0;
-        if (host) HIP_TRY(hipHostMalloc((void **)&b.hdata, (size_t)b.cells * 4, hipHostMallocDefault), "pinned buffer");
-        HIP_TRY(hipMalloc((void **)&b.ddata, (size_t)b.cells * 4), "tile buffer");
+        int e = STRATA_OK;
+        if (host && (e = PinnedCache::get().acquire((size_t)b.cells * 4, (void **)&b.hdata))) return e;
+        if ((e = pool_alloc((void **)&b.ddata, (size_t)b.cells * 4, st, "tile buffer"))) return e;
         if (m) {
-            if (host)
-                HIP_TRY(hipHostMalloc((void **)&b.hmask, (size_t)b.words * 8, hipHostMallocDefault), "pinned mask");
-            HIP_TRY(hipMalloc((void **)&b.dmask, (size_t)b.words * 8), "tile mask");
+            if (host && (e = PinnedCache::get().acquire((size_t)b.words * 8, (void **)&b.hmask))) return e;
+            if ((e = pool_alloc((void **)&b.dmask, (size_t)b.words * 8, st, "tile mask"))) return e;
         }
         return STRATA_OK;
     };
@@ -1580,7 +1586,7 @@ int strata_reduce(int what, const strata_raster *src, const strata_options *opts
     }
     FoldScratch fs;
     fs.st = st;
-    HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+    if ((rc = pool_alloc((void **)&fs.dev, FoldScratch::bytes(), st, "reduction scratch"))) return rc;
     HIP_TRY(launch_fold(fold_args(*d, what, d->valid != nullptr), fs.bins(), fs.parts(), fs.total(), true, st),
             "launching the fold");
     if ((rc = fold_read(fs, what, out))) return rc;
@@ -1688,7 +1694,7 @@ int strata_reduce_chunked(int what, const strata_source *src, const strata_optio
         // Read in place: one launch over the raster, the tiles' counters in
         // closed form.
         if (cancelled(opts)) return STRATA_CANCELLED;
-        HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+        if ((rc = pool_alloc((void **)&fs.dev, FoldScratch::bytes(), st, "reduction scratch"))) return rc;
         HIP_TRY(launch_fold(fold_args(*src->resident, what, src->masked && src->resident->valid), fs.bins(),
                             fs.parts(), fs.total(), true, st),
                 "launching the fold");
@@ -1710,21 +1716,23 @@ int strata_reduce_chunked(int what, const strata_source *src, const strata_optio
         struct Bufs {
             float *&hd, *&dd;
             uint64_t *&hm, *&dm;
+            hipStream_t st;
             ~Bufs()
             {
-                if (hd) (void)hipHostFree(hd);
-                if (hm) (void)hipHostFree(hm);
-                if (dd) (void)hipFree(dd);
-                if (dm) (void)hipFree(dm);
+                (void)hipStreamSynchronize(st);
+                PinnedCache::get().release(hd);
+                PinnedCache::get().release(hm);
+                if (dd) (void)hipFreeAsync(dd, st);
+                if (dm) (void)hipFreeAsync(dm, st);
             }
-        } bufs{hdata, ddata, hmask, dmask};
-        HIP_TRY(hipHostMalloc((void **)&hdata, (size_t)cells * 4, hipHostMallocDefault), "pinned buffer");
-        HIP_TRY(hipMalloc((void **)&ddata, (size_t)cells * 4), "tile buffer");
+        } bufs{hdata, ddata, hmask, dmask, st};
+        if ((rc = PinnedCache::get().acquire((size_t)cells * 4, (void **)&hdata))) return rc;
+        if ((rc = pool_alloc((void **)&ddata, (size_t)cells * 4, st, "tile buffer"))) return rc;
         if (masked) {
-            HIP_TRY(hipHostMalloc((void **)&hmask, (size_t)words * 8, hipHostMallocDefault), "pinned mask");
-            HIP_TRY(hipMalloc((void **)&dmask, (size_t)words * 8), "tile mask");
+            if ((rc = PinnedCache::get().acquire((size_t)words * 8, (void **)&hmask))) return rc;
+            if ((rc = pool_alloc((void **)&dmask, (size_t)words * 8, st, "tile mask"))) return rc;
         }
-        HIP_TRY(hipMallocAsync((void **)&fs.dev, FoldScratch::bytes(), st), "reduction scratch");
+        if ((rc = pool_alloc((void **)&fs.dev, FoldScratch::bytes(), st, "reduction scratch"))) return rc;
         bool first = true;
         for (int64_t t = 0; t < g.tiles; ++t) {
             if (cancelled(opts)) {
@@ -1827,3 +1835,14 @@ int strata_process_fanout_chunked(const strata_kernel *parts, int nparts, const 
 }
 
 } // extern "C"
+
+namespace strata {
+
+hipError_t scratch_alloc_async(void **p, size_t bytes, hipStream_t st)
+{
+    hipMemPool_t pool = nullptr;
+    if (scratch_pool(&pool) != STRATA_OK) return hipErrorOutOfMemory;
+    return hipMallocFromPoolAsync(p, bytes, pool, st);
+}
+
+} // namespace strata

@@ -287,9 +287,6 @@ def check(dst: List[Float32Raster], src: List[Float32Raster], k: KernelDesc, opt
             raise EnginePanic(
                 f"engine: an input has a validity mask but dst[{i}].Valid is nil; allocate it with "
                 "raster.NewFloat32Like, or set Valid on its root raster before windowing")
-    devs = {bool(x.on_device) for x in list(dst) + list(src)}
-    if len(devs) > 1:
-        raise EnginePanic("engine: operands must all be in device memory or all in host memory")
     for i, d in enumerate(dst):
         for j in range(i + 1, len(dst)):
             meet = data_spans(d, dst[j]) if r > 0 else data_relation(d, dst[j]) != DISJOINT
