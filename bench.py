@@ -15,9 +15,10 @@ profile/plan/mean Curvature, Gradient) on the same DEM, a batch of 4096
 uneven mixed-op tiles run as one persistent graph-captured launch against
 one launch set per tile, a serving sweep (Poisson arrivals at fixed QPS,
 p50/p99 latency of the chain Slope -> Lookup -> Reclass on window
-requests), and the CPU baseline: the oracle (a scalar C restatement of the reference's kernels)
-timed on the host cores over a bounded sample of the same rows, whose
-output is also compared bit for bit with the GPU's.
+requests), and the CPU baseline: the reference's AVX2 Tiled path restated
+in C (oracle/strata_oracle.c), timed on up to 16 host cores over the whole
+DEM for Slope and Hillshade and on one core for the reference's plain
+4096x4096 Slope, whose outputs are compared bit for bit with the GPU's.
 
 Prints one JSON line.
 """
@@ -54,8 +55,8 @@ def parse():
     ap.add_argument("--suite", choices=["terrain", "slope"], default="terrain",
                     help="terrain: Slope then the rest of the suite; slope: Slope only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-rows", type=int, default=2000, help="rows in the CPU baseline sample")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work (thread-seconds) to aim for")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU work (thread-seconds) to aim for in each timed baseline op")
     ap.add_argument("--sweep-rows", type=int, default=0,
                     help="rows per block of the radius-1 kernels (0: the library's default)")
     ap.add_argument("--batch-tiles", type=int, default=4096,
@@ -755,48 +756,57 @@ def serving_bench(args, dem, N, log):
 
 
 def cpu_baseline(args, dem, out, slope, N, log):
-    """The oracle timed on the host cores over the first rows of the same
-    DEM; its output is compared with the GPU's on the rows both compute
-    as interior."""
+    """The CPU baseline: the reference's AVX2 Tiled path restated in C
+    (oracle/strata_oracle.c orc_tiled_avx2: 8-cell lanes in
+    internal/stencil/simd_amd64.go's operation order, bands of
+    max(1, 65536/W) rows claimed in order by worker threads as
+    internal/exec/worker.go:144-181), timed on the host cores over the
+    whole DEM for Slope and Hillshade. The GPU's Slope output is compared
+    with it bit for bit over the whole raster."""
     import numpy as np
+    import torch
 
     from oracle import oracle
+    from strata_amd import terrain
     from strata_amd.raster import Float32Raster
 
-    # First, while no oracle threads are still spinning after a parallel run
+    # First, while no worker threads are still running after a parallel run
     # (they would take the cores the launching thread needs).
     plain = plain_4096(slope, log)
 
-    rows = min(args.cpu_rows, N)
-    n = rows * N
-    words = (n + 63) // 64
-    src = Float32Raster(dem.Data[:n].cpu().numpy().copy(), N, rows, N,
-                        dem.Valid[:words].cpu().numpy().view(np.uint64).copy(), 0)
-    dst = Float32Raster(np.zeros(n, np.float32), N, rows, N, np.zeros(words, np.uint64), 0)
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    t = time.perf_counter()
-    oracle.process(slope, [dst], [src], threads=threads)
-    one = time.perf_counter() - t
-    reps = max(1, min(int(args.cpu_seconds / max(one * threads, 1e-6)), 200))
-    t = time.perf_counter()
-    for _ in range(reps):
-        oracle.process(slope, [dst], [src], threads=threads)
-    el = time.perf_counter() - t
-    rate = n * reps / el / 1e9
-    log(f"bench: CPU oracle {threads} threads, {rows}x{N} sample x{reps}: {rate:.3f} Gcells/s")
+    n = N * N
+    src = Float32Raster(dem.Data.cpu().numpy(), N, N, N, dem.Valid.cpu().numpy().view(np.uint64), 0)
+    dst = Float32Raster(np.empty(n, np.float32), N, N, N, np.zeros(len(src.Valid), np.uint64), 0)
 
-    # Rows 1..rows-2 are interior in both the sample and the full raster.
-    got = out.Data[N:(rows - 1) * N].cpu().numpy()
-    want = dst.Data[N:(rows - 1) * N]
-    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
-    from strata_amd.raster import unpack_bits
-    gv = unpack_bits(out.Valid, N, (rows - 2) * N)
-    wv = unpack_bits(dst.Valid, N, (rows - 2) * N)
-    parity = {"rows": rows - 2, "cells": int(same.size), "data_equal": bool(same.all()),
-              "valid_equal": bool(np.array_equal(gv, wv))}
-    cpu = {"value": round(rate, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
-           "sample": f"oracle (scalar C restatement) Slope on rows 0-{rows - 1} of the same DEM ({rows}x{N}), "
-                     f"{reps} passes, {threads} threads"}
+    def timed(k):
+        t = time.perf_counter()
+        oracle.tiled_avx2(k, [dst], [src], 0, 0, threads)
+        one = time.perf_counter() - t
+        reps = max(1, min(int(args.cpu_seconds / max(one * threads, 1e-6)), 50))
+        t = time.perf_counter()
+        for _ in range(reps):
+            oracle.tiled_avx2(k, [dst], [src], 0, 0, threads)
+        el = (time.perf_counter() - t) / reps
+        return el, reps
+
+    hill = terrain.hillshadeOp(terrain.HillshadeOptions(CellSize=30, Azimuth=315, Altitude=45))
+    h_s, h_reps = timed(hill)
+    s_s, s_reps = timed(slope)  # last, so dst holds Slope for the comparison
+    log(f"bench: CPU AVX2 Tiled {threads} threads, {N}x{N}: Slope {s_s * 1e3:.1f} ms ({n / s_s / 1e9:.3f} Gcells/s), "
+        f"Hillshade {h_s * 1e3:.1f} ms ({n / h_s / 1e9:.3f} Gcells/s)")
+
+    got = out.Data.cpu().numpy()
+    same = (got.view(np.uint32) == dst.Data.view(np.uint32)) | (np.isnan(got) & np.isnan(dst.Data))
+    gv = out.Valid.cpu().numpy().view(np.uint64)
+    parity = {"cells": int(n), "data_equal": bool(same.all()), "valid_equal": bool(np.array_equal(gv, dst.Valid)),
+              "against": "the CPU baseline's Slope of the whole DEM"}
+    cpu = {"value": round(n / s_s / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+           "sample": f"the whole {N}x{N} DEM, terrain.Slope (CellSize 30, degrees) through a C restatement of the "
+                     f"reference's AVX2 Tiled path, {threads} threads, {s_reps} timed passes",
+           "hillshade": {"value": round(n / h_s / 1e9, 4), "unit": "Gcells/s", "ms": round(h_s * 1e3, 2),
+                         "passes": h_reps},
+           "slope_ms": round(s_s * 1e3, 2)}
     cpu["plain_4096"] = plain
     return cpu, parity
 
@@ -804,9 +814,11 @@ def cpu_baseline(args, dem, out, slope, N, log):
 def plain_4096(slope, log):
     """The reference's CPU-runnable configuration: plain terrain.Slope
     (CellSize 30, degrees) on a 4096x4096 fBm DEM with 1% of cells NoData
-    i.i.d. The oracle runs it on one thread, as the plain path does; the
-    device runs the same call (strata_process); the two outputs, data and
-    validity, are compared bit for bit over the whole raster."""
+    i.i.d. The CPU baseline (the reference's AVX2 path restated) runs it on
+    one thread, as the plain path does, beside the reference's published
+    one-core figure; the device runs the same call (strata_process); the
+    two outputs, data and validity, are compared bit for bit over the whole
+    raster."""
     import numpy as np
     import torch
 
@@ -835,22 +847,26 @@ def plain_4096(slope, log):
 
     src = Float32Raster(data.cpu().numpy().copy(), n, n, n, valid.cpu().numpy().view(np.uint64).copy(), 0)
     dst = Float32Raster(np.zeros(n * n, np.float32), n, n, n, np.zeros(words, np.uint64), 0)
+    oracle.tiled_avx2(slope, [dst], [src], 0, 0, 1)
+    cpu_reps = 10
     t = time.perf_counter()
-    oracle.process(slope, [dst], [src], threads=1)
-    cpu_s = time.perf_counter() - t
+    for _ in range(cpu_reps):
+        oracle.tiled_avx2(slope, [dst], [src], 0, 0, 1)
+    cpu_s = (time.perf_counter() - t) / cpu_reps
 
     got = out.Data.cpu().numpy()
     same = (got.view(np.uint32) == dst.Data.view(np.uint32)) | (np.isnan(got) & np.isnan(dst.Data))
     gv = out.Valid.cpu().numpy().view(np.uint64)
     res = {"workload": "plain terrain.Slope, 4096x4096 fBm DEM (Hurst 0.8, 0-3000 m), 1% NoData i.i.d., CellSize 30",
            "cpu_1_thread_s": round(cpu_s, 4), "cpu_1_thread_gcells_s": round(n * n / cpu_s / 1e9, 4),
+           "cpu_kind": "the reference's AVX2 path restated in C, one thread",
+           "reference_published_1_core_gcells_s": 0.688,
            "gpu_ms": round(gpu_s * 1e3, 4), "gpu_gcells_s": round(n * n / gpu_s / 1e9, 2),
            "gpu_over_cpu": round(cpu_s / gpu_s, 1),
            "identical": bool(same.all() and np.array_equal(gv, dst.Valid))}
-    log(f"bench: plain Slope 4096^2: CPU 1 thread {cpu_s:.3f} s, GPU {gpu_s * 1e3:.3f} ms, identical "
-        f"{res['identical']}")
+    log(f"bench: plain Slope 4096^2: CPU AVX2 1 thread {cpu_s * 1e3:.1f} ms ({n * n / cpu_s / 1e9:.3f} Gcells/s), "
+        f"GPU {gpu_s * 1e3:.3f} ms, identical {res['identical']}")
     return res
-
 
 if __name__ == "__main__":
     sys.exit(main())

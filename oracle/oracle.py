@@ -56,6 +56,9 @@ def lib():
         L.orc_process_rows.restype = c_int
         L.orc_correlate_f64.argtypes = [POINTER(c_double), c_int64, POINTER(OrcRaster), POINTER(c_float), c_int]
         L.orc_correlate_f64.restype = c_int
+        L.orc_tiled_avx2.argtypes = [POINTER(OrcKernel), POINTER(OrcRaster), c_int, POINTER(OrcRaster), c_int,
+                                     c_int64, c_int64, c_int]
+        L.orc_tiled_avx2.restype = c_int
         _lib = L
     return _lib
 
@@ -102,6 +105,23 @@ def process(k, dst, src, y0: int = 0, y1: int = None, threads: int = 1) -> None:
         y1 = src[0].Height
     rc = lib().orc_process_rows(ctypes.byref(kc), cd, len(dst), cs, len(src), y0, y1, threads)
     del taps
+    if rc != 0:
+        raise ValueError(f"oracle: bad kernel or operands for {k}")
+
+
+def tiled_avx2(k, dst, src, tile_w: int = 0, tile_h: int = 0, threads: int = 1) -> None:
+    """The CPU baseline: the reference's AVX2 Tiled path (radius-1
+    Gradient, Slope, Hillshade) over its band plan with `threads` workers,
+    into host rasters dst. Same bits as process()."""
+    kc, taps = _kernel(k)
+    cd = (OrcRaster * len(dst))(*[_raster(d) for d in dst])
+    cs = (OrcRaster * len(src))(*[_raster(s) for s in src])
+    rc = lib().orc_tiled_avx2(ctypes.byref(kc), cd, len(dst), cs, len(src), tile_w, tile_h, threads)
+    del taps
+    if rc == -2:
+        raise ValueError(f"oracle: the AVX2 baseline does not cover {k}")
+    if rc == -3:
+        raise RuntimeError("oracle: this CPU has no AVX2")
     if rc != 0:
         raise ValueError(f"oracle: bad kernel or operands for {k}")
 

@@ -542,3 +542,308 @@ int orc_correlate_f64(double *dst, int64_t dst_stride, const orc_raster *s, cons
         }
     return 0;
 }
+
+/* =====================================================================
+ * CPU baseline: the reference's AVX2 Tiled path, restated.
+ *
+ * Lanes (8 cells per 256-bit vector), internal/stencil/simd_amd64.go:
+ *   hornDiff8 :74-84, hornGradientLanes :92-106, hornMagnitude8 and
+ *   magnitude8 :119-128, hornSlopeAtanLanes :136-148, atan8 :179-192
+ *   (both reductions computed, selected in Atan32's order),
+ *   hornHillshadeLanes and shade8 :284-305; each lane runs the scalar
+ *   formula operation for operation (IEEE sqrt and divide, no fused
+ *   multiply-add), the row's last n % 8 cells run the scalar kernel.
+ * Schedule: the band plan of internal/exec/tile.go:298-359 (tiles row
+ *   major, bands of max(1, 65536/tileW) rows), claimed in order by
+ *   worker threads from one counter (internal/exec/worker.go:144-181).
+ * Validity, word by word as internal/stencil/mask.go:55-160 (ErodeBox at
+ *   radius 1: the AND of the three source rows, shrunk by two cells),
+ *   over the band's interior; the engine's edge ring gets cleared bits
+ *   (internal/exec/halo.go:161-290). Words a band shares with another
+ *   band are updated atomically on its own bits.
+ * Radius-1 Gradient, Slope and Hillshade. The results equal the per-cell
+ * functions above bit for bit (tests/test_oracle_avx2.py).
+ * ===================================================================== */
+#include <immintrin.h>
+
+#define ORC_AVX2 __attribute__((target("avx2")))
+
+ORC_AVX2 static inline __m256 atan8(__m256 x)
+{
+    const __m256 one = _mm256_set1_ps(1.0f);
+    const __m256 mid = _mm256_cmp_ps(x, _mm256_set1_ps(ATAN_TANPI8), _CMP_GT_OQ);
+    const __m256 big = _mm256_cmp_ps(x, _mm256_set1_ps(ATAN_TAN3PI8), _CMP_GT_OQ);
+    __m256 num = _mm256_blendv_ps(x, _mm256_sub_ps(x, one), mid);
+    num = _mm256_blendv_ps(num, _mm256_set1_ps(-1.0f), big);
+    __m256 den = _mm256_blendv_ps(one, _mm256_add_ps(x, one), mid);
+    den = _mm256_blendv_ps(den, x, big);
+    __m256 y0 = _mm256_blendv_ps(_mm256_setzero_ps(), _mm256_set1_ps(u2f(ATAN_PI4_BITS)), mid);
+    y0 = _mm256_blendv_ps(y0, _mm256_set1_ps(u2f(ATAN_PI2_BITS)), big);
+    const __m256 t = _mm256_div_ps(num, den);
+    const __m256 z = _mm256_mul_ps(t, t);
+    __m256 p = _mm256_sub_ps(_mm256_mul_ps(_mm256_set1_ps(ATAN_C4), z), _mm256_set1_ps(ATAN_C3));
+    p = _mm256_add_ps(_mm256_mul_ps(p, z), _mm256_set1_ps(ATAN_C2));
+    p = _mm256_sub_ps(_mm256_mul_ps(p, z), _mm256_set1_ps(ATAN_C1));
+    return _mm256_add_ps(y0, _mm256_add_ps(_mm256_mul_ps(_mm256_mul_ps(p, z), t), t));
+}
+
+/* Horn's unscaled differences of eight adjacent cells; rows r0..r2 start
+ * at the column left of the first cell. */
+ORC_AVX2 static inline void horn_diff8(const float *r0, const float *r1, const float *r2, __m256 *dx, __m256 *dy)
+{
+    const __m256 z1 = _mm256_loadu_ps(r0), z2 = _mm256_loadu_ps(r0 + 1), z3 = _mm256_loadu_ps(r0 + 2);
+    const __m256 z4 = _mm256_loadu_ps(r1), z6 = _mm256_loadu_ps(r1 + 2);
+    const __m256 z7 = _mm256_loadu_ps(r2), z8 = _mm256_loadu_ps(r2 + 1), z9 = _mm256_loadu_ps(r2 + 2);
+    __m256 d = _mm256_sub_ps(z6, z4);
+    *dx = _mm256_add_ps(_mm256_add_ps(_mm256_sub_ps(z3, z1), _mm256_sub_ps(z9, z7)), _mm256_add_ps(d, d));
+    d = _mm256_sub_ps(z8, z2);
+    *dy = _mm256_add_ps(_mm256_add_ps(_mm256_sub_ps(z7, z1), _mm256_sub_ps(z9, z3)), _mm256_add_ps(d, d));
+}
+
+/* The op over cells [0, n) of a row (window rows r0..r2 from the column
+ * left of cell 0) in whole lanes; returns the cells written. */
+ORC_AVX2 static int64_t horn_lanes(const orc_kernel *k, float *o0, float *o1, const float *r0, const float *r1,
+                                   const float *r2, int64_t n)
+{
+    const __m256 kx = _mm256_set1_ps(k->k[0]), ky = _mm256_set1_ps(k->k[1]);
+    int64_t i = 0;
+    __m256 dx, dy;
+    switch (k->op) {
+    case ORC_OP_GRADIENT:
+        for (; i + 8 <= n; i += 8) {
+            horn_diff8(r0 + i, r1 + i, r2 + i, &dx, &dy);
+            _mm256_storeu_ps(o0 + i, _mm256_mul_ps(dx, kx));
+            _mm256_storeu_ps(o1 + i, _mm256_mul_ps(dy, ky));
+        }
+        break;
+    case ORC_OP_SLOPE: {
+        const __m256 scale = _mm256_set1_ps(k->k[2]);
+        const int atan_ = (k->flags & ORC_FLAG_ATAN) != 0;
+        for (; i + 8 <= n; i += 8) {
+            horn_diff8(r0 + i, r1 + i, r2 + i, &dx, &dy);
+            const __m256 gx = _mm256_mul_ps(dx, kx), gy = _mm256_mul_ps(dy, ky);
+            __m256 m = _mm256_sqrt_ps(_mm256_add_ps(_mm256_mul_ps(gx, gx), _mm256_mul_ps(gy, gy)));
+            if (atan_) m = atan8(m);
+            _mm256_storeu_ps(o0 + i, _mm256_mul_ps(m, scale));
+        }
+        break;
+    }
+    case ORC_OP_HILLSHADE: {
+        const __m256 c = _mm256_set1_ps(k->k[2]), bx = _mm256_set1_ps(k->k[3]), by = _mm256_set1_ps(k->k[4]);
+        const __m256 zero = _mm256_setzero_ps(), hi = _mm256_set1_ps(255.0f), one = _mm256_set1_ps(1.0f);
+        for (; i + 8 <= n; i += 8) {
+            horn_diff8(r0 + i, r1 + i, r2 + i, &dx, &dy);
+            const __m256 gx = _mm256_mul_ps(dx, kx), gy = _mm256_mul_ps(dy, ky);
+            const __m256 num = _mm256_add_ps(c, _mm256_add_ps(_mm256_mul_ps(bx, gx), _mm256_mul_ps(by, gy)));
+            const __m256 den =
+                _mm256_sqrt_ps(_mm256_add_ps(one, _mm256_add_ps(_mm256_mul_ps(gx, gx), _mm256_mul_ps(gy, gy))));
+            __m256 v = _mm256_div_ps(num, den);
+            v = _mm256_blendv_ps(v, zero, _mm256_cmp_ps(v, zero, _CMP_LT_OQ));
+            v = _mm256_blendv_ps(v, hi, _mm256_cmp_ps(v, hi, _CMP_GT_OQ));
+            _mm256_storeu_ps(o0 + i, v);
+        }
+        break;
+    }
+    }
+    return i;
+}
+
+/* Word-level bit helpers of the baseline's validity (mask.go:183-303). */
+static void bits_extract(uint64_t *acc, const uint64_t *m, int64_t from, int64_t n)
+{
+    const int64_t nw = (n + 63) >> 6, w0 = from >> 6, last = (from + n - 1) >> 6;
+    const int sh = (int)(from & 63);
+    for (int64_t i = 0; i < nw; i++) {
+        uint64_t v = m[w0 + i] >> sh;
+        if (sh && w0 + i + 1 <= last) v |= m[w0 + i + 1] << (64 - sh);
+        acc[i] = v;
+    }
+}
+static void bits_and(uint64_t *acc, const uint64_t *m, int64_t from, int64_t n)
+{
+    const int64_t nw = (n + 63) >> 6, w0 = from >> 6, last = (from + n - 1) >> 6;
+    const int sh = (int)(from & 63);
+    for (int64_t i = 0; i < nw; i++) {
+        uint64_t v = m[w0 + i] >> sh;
+        if (sh && w0 + i + 1 <= last) v |= m[w0 + i + 1] << (64 - sh);
+        acc[i] &= v;
+    }
+}
+/* Cell i becomes the AND of cells i, i+1, i+2 (shrink2Row). */
+static void bits_shrink2(uint64_t *acc, int64_t nw)
+{
+    for (int64_t i = 0; i < nw; i++) {
+        const uint64_t a = acc[i], b = i + 1 < nw ? acc[i + 1] : 0;
+        acc[i] = a & ((a >> 1) | (b << 63)) & ((a >> 2) | (b << 62));
+    }
+}
+/* Bits [to, to+n) of m from acc (acc == NULL: all set to v); words only
+ * partly in the range are updated atomically. */
+static void bits_deposit(uint64_t *m, int64_t to, const uint64_t *acc, int64_t n, int v)
+{
+    if (n <= 0) return;
+    const int64_t w0 = to >> 6, w1 = (to + n - 1) >> 6, nsw = (n + 63) >> 6;
+    for (int64_t w = w0; w <= w1; w++) {
+        const int64_t lo = (to > 64 * w ? to : 64 * w) - 64 * w;
+        const int64_t hi = (to + n < 64 * w + 64 ? to + n : 64 * w + 64) - 64 * w;
+        const uint64_t own = hi - lo == 64 ? ~(uint64_t)0 : ((((uint64_t)1 << (hi - lo)) - 1) << lo);
+        uint64_t bits;
+        if (!acc) {
+            bits = v ? ~(uint64_t)0 : 0;
+        } else {
+            const int64_t sb = 64 * w - to;
+            if (sb < 0) {
+                bits = acc[0] << (-sb);
+            } else {
+                const int64_t q = sb >> 6;
+                const int s = (int)(sb & 63);
+                bits = acc[q] >> s;
+                if (s && q + 1 < nsw) bits |= acc[q + 1] << (64 - s);
+            }
+        }
+        bits &= own;
+        if (own == ~(uint64_t)0) {
+            m[w] = bits;
+        } else {
+            __atomic_fetch_and(&m[w], ~own | bits, __ATOMIC_RELAXED);
+            __atomic_fetch_or(&m[w], bits, __ATOMIC_RELAXED);
+        }
+    }
+}
+
+typedef struct {
+    const orc_kernel *k;
+    orc_raster *dst;
+    int ndst;
+    const orc_raster *src;
+    int64_t tileW, tileH, bandRows, tilesX, tileRows, perTile, perLastTile, bands;
+    int64_t next; /* claim counter */
+} tiled_job;
+
+/* plan.band (internal/exec/tile.go:343-359) */
+static void tiled_band(const tiled_job *j, int64_t i, int64_t *x0, int64_t *y0, int64_t *x1, int64_t *y1)
+{
+    const int64_t W = j->src->width, H = j->src->height;
+    const int64_t perRow = j->tilesX * j->perTile;
+    int64_t tr = i / perRow, q = i % perRow, per = j->perTile;
+    if (tr >= j->tileRows - 1) {
+        tr = j->tileRows - 1;
+        q = i - (j->tileRows - 1) * perRow;
+        per = j->perLastTile;
+    }
+    const int64_t tx = q / per, b = q % per, ty = tr * j->tileH;
+    *x0 = tx * j->tileW;
+    *x1 = *x0 + j->tileW < W ? *x0 + j->tileW : W;
+    *y0 = ty + b * j->bandRows;
+    int64_t e = *y0 + j->bandRows;
+    if (e > ty + j->tileH) e = ty + j->tileH;
+    *y1 = e < H ? e : H;
+}
+
+ORC_AVX2 static void tiled_band_run(const tiled_job *j, int64_t x0, int64_t y0, int64_t x1, int64_t y1, uint64_t *acc)
+{
+    const orc_kernel *k = j->k;
+    const orc_raster *s = j->src;
+    const int64_t W = s->width, H = s->height;
+    const int64_t ix0 = x0 > 1 ? x0 : 1, ix1 = x1 < W - 1 ? x1 : W - 1;
+    const float nan = edge_nan();
+    for (int64_t y = y0; y < y1; y++) {
+        float *o0 = j->dst[0].data + y * j->dst[0].stride;
+        float *o1 = j->ndst > 1 ? j->dst[1].data + y * j->dst[1].stride : NULL;
+        const int row_in = y >= 1 && y < H - 1 && ix0 < ix1;
+        for (int64_t x = x0; x < x1; x++) {
+            if (row_in && x >= ix0 && x < ix1) {
+                x = ix1 - 1;
+                continue;
+            }
+            o0[x] = nan;
+            if (o1) o1[x] = nan;
+        }
+        if (!row_in) continue;
+        const float *r0 = s->data + (y - 1) * s->stride + (ix0 - 1);
+        const float *r1 = r0 + s->stride, *r2 = r1 + s->stride;
+        const int64_t n = ix1 - ix0;
+        int64_t i = horn_lanes(k, o0 + ix0, o1 ? o1 + ix0 : NULL, r0, r1, r2, n);
+        for (; i < n; i++) {
+            window_t w = {r0 + i, s->stride};
+            float out[2];
+            horn_cell(k, w, out);
+            o0[ix0 + i] = out[0];
+            if (o1) o1[ix0 + i] = out[1];
+        }
+    }
+    for (int o = 0; o < j->ndst; o++) {
+        orc_raster *d = &j->dst[o];
+        if (!d->valid) continue;
+        for (int64_t y = y0; y < y1; y++) {
+            const int64_t row = d->valid_offset + y * d->stride;
+            if (y < 1 || y >= H - 1 || ix0 >= ix1) {
+                bits_deposit(d->valid, row + x0, NULL, x1 - x0, 0);
+                continue;
+            }
+            if (x0 < ix0) bits_deposit(d->valid, row + x0, NULL, ix0 - x0, 0);
+            if (ix1 < x1) bits_deposit(d->valid, row + ix1, NULL, x1 - ix1, 0);
+            const int64_t n = ix1 - ix0;
+            if (!s->valid) {
+                bits_deposit(d->valid, row + ix0, NULL, n, 1);
+                continue;
+            }
+            const int64_t sb = s->valid_offset + (y - 1) * s->stride + (ix0 - 1);
+            bits_extract(acc, s->valid, sb, n + 2);
+            bits_and(acc, s->valid, sb + s->stride, n + 2);
+            bits_and(acc, s->valid, sb + 2 * s->stride, n + 2);
+            bits_shrink2(acc, (n + 2 + 63) >> 6);
+            bits_deposit(d->valid, row + ix0, acc, n, 0);
+        }
+    }
+}
+
+static void *tiled_worker(void *arg)
+{
+    tiled_job *j = (tiled_job *)arg;
+    uint64_t *acc = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)((j->src->width + 2 + 63) / 64 + 1));
+    if (!acc) return NULL;
+    for (;;) {
+        const int64_t i = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
+        if (i >= j->bands) break;
+        int64_t x0, y0, x1, y1;
+        tiled_band(j, i, &x0, &y0, &x1, &y1);
+        tiled_band_run(j, x0, y0, x1, y1, acc);
+    }
+    free(acc);
+    return NULL;
+}
+
+int orc_tiled_avx2(const orc_kernel *k, orc_raster *dst, int ndst, const orc_raster *src, int nsrc, int64_t tile_w,
+                   int64_t tile_h, int nthreads)
+{
+    if (!k || !dst || !src || check_kernel(k, ndst, nsrc) != 0) return -1;
+    if (k->radius != 1 || (k->flags & ORC_FLAG_FIT) ||
+        (k->op != ORC_OP_GRADIENT && k->op != ORC_OP_SLOPE && k->op != ORC_OP_HILLSHADE))
+        return -2;
+    if (!__builtin_cpu_supports("avx2")) return -3;
+    const int64_t W = src[0].width, H = src[0].height;
+    for (int o = 0; o < ndst; o++)
+        if (dst[o].width != W || dst[o].height != H) return -1;
+    if (src[0].valid)
+        for (int o = 0; o < ndst; o++)
+            if (!dst[o].valid) return -1;
+    tiled_job j = {k, dst, ndst, src, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    j.tileW = tile_w > 0 && tile_w < W ? tile_w : W;
+    j.tileH = tile_h > 0 && tile_h < H ? tile_h : H;
+    j.bandRows = 65536 / j.tileW > 1 ? 65536 / j.tileW : 1;
+    j.tilesX = (W + j.tileW - 1) / j.tileW;
+    j.tileRows = (H + j.tileH - 1) / j.tileH;
+    j.perTile = (j.tileH + j.bandRows - 1) / j.bandRows;
+    j.perLastTile = (H - (j.tileRows - 1) * j.tileH + j.bandRows - 1) / j.bandRows;
+    j.bands = j.tilesX * ((j.tileRows - 1) * j.perTile + j.perLastTile);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    int used = 0;
+    for (int t = 1; t < nthreads; t++)
+        if (pthread_create(&th[used], NULL, tiled_worker, &j) == 0) used++;
+    tiled_worker(&j);
+    for (int t = 0; t < used; t++) pthread_join(th[t], NULL);
+    return 0;
+}

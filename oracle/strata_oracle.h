@@ -78,6 +78,14 @@ int orc_process(const orc_kernel *k, orc_raster *dst, int ndst, const orc_raster
 int orc_process_rows(const orc_kernel *k, orc_raster *dst, int ndst, const orc_raster *src, int nsrc,
                      int64_t y0, int64_t y1, int nthreads);
 
+/* The CPU baseline: the reference's AVX2 Tiled path for radius-1
+ * Gradient, Slope and Hillshade (no fit), its band plan for tile_w x
+ * tile_h tiles (0: the raster's size) claimed in order by nthreads
+ * threads. Same results as orc_process. Returns 0, -1 on a bad call, -2
+ * for an op it does not cover, -3 without AVX2. */
+int orc_tiled_avx2(const orc_kernel *k, orc_raster *dst, int ndst, const orc_raster *src, int nsrc, int64_t tile_w,
+                   int64_t tile_h, int nthreads);
+
 /* FP64 correlation used as the accuracy yardstick for approximate paths:
  * dst[i] = sum over the (2r+1)^2 window of w*v in double. */
 int orc_correlate_f64(double *dst, int64_t dst_stride, const orc_raster *src, const float *w, int r);
