@@ -300,6 +300,32 @@ int strata_plan_stats_fanout(const strata_kernel *parts, int nparts, int64_t wid
                              const strata_options *opts, int chunked, strata_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Chains: a pipeline of stages, each reading the one before it        */
+/* ------------------------------------------------------------------ */
+/*
+ * exec.ProcessN / exec.ProcessChunked with the kernel exec.NewPipeline(1,
+ * stages, [last]) where stage i reads stage i-1's value and stage 0 the
+ * input (internal/exec/pipeline.go:166-187, :539; the chains graph.Slope,
+ * graph.Lookup, graph.Reclass lower to, graph/graph.go:250-287). The
+ * output is bit for bit what the stages write as separate calls, each on
+ * the one before: its edge ring and its validity erosion are the chain's
+ * reach, the sum of the stage radii (pipeline.go:32-41, 83-114), and its
+ * Stats are those of one kernel of that radius. Point stages (Lookup,
+ * Reclass) after a radius-1 Horn stage are applied to its values in
+ * registers before the store; any other stage writes its values once, into
+ * device scratch the next stage reads. Every stage has one input and one
+ * output; at most 8 stages and a reach of at most 8. Errors as
+ * strata_process.
+ */
+int strata_process_chain(const strata_kernel *stages, int nstages, const strata_raster *dst, const strata_raster *src,
+                         const strata_options *opts, strata_stats *stats);
+int strata_process_chain_chunked(const strata_kernel *stages, int nstages, const strata_sink *dst,
+                                 const strata_source *src, const strata_options *opts, strata_stats *stats);
+/* Closed-form Stats of a chain call, as strata_plan_stats. */
+int strata_plan_stats_chain(const strata_kernel *stages, int nstages, int64_t width, int64_t height,
+                            const strata_options *opts, int chunked, strata_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Reductions: exec.Reduce / exec.ReduceChunked with the reduce        */
 /* package's reducers                                                  */
 /* ------------------------------------------------------------------ */

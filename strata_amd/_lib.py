@@ -265,6 +265,9 @@ EXPORTS = (
     "strata_process_fanout",
     "strata_process_fanout_chunked",
     "strata_plan_stats_fanout",
+    "strata_process_chain",
+    "strata_process_chain_chunked",
+    "strata_plan_stats_chain",
     "strata_reduce",
     "strata_reduce_chunked",
     "strata_reduce_noalloc",
@@ -330,6 +333,15 @@ def lib() -> ctypes.CDLL:
     L.strata_plan_stats_fanout.argtypes = [POINTER(Kernel), c_int, c_int64, c_int64, POINTER(Options), c_int,
                                            POINTER(Stats)]
     L.strata_plan_stats_fanout.restype = c_int
+    L.strata_process_chain.argtypes = [POINTER(Kernel), c_int, POINTER(Raster), POINTER(Raster),
+                                       POINTER(Options), POINTER(Stats)]
+    L.strata_process_chain.restype = c_int
+    L.strata_process_chain_chunked.argtypes = [POINTER(Kernel), c_int, POINTER(Sink), POINTER(Source),
+                                               POINTER(Options), POINTER(Stats)]
+    L.strata_process_chain_chunked.restype = c_int
+    L.strata_plan_stats_chain.argtypes = [POINTER(Kernel), c_int, c_int64, c_int64, POINTER(Options), c_int,
+                                          POINTER(Stats)]
+    L.strata_plan_stats_chain.restype = c_int
     L.strata_reduce.argtypes = [c_int, POINTER(Raster), POINTER(Options), POINTER(Fold), POINTER(Stats)]
     L.strata_reduce.restype = c_int
     L.strata_reduce_chunked.argtypes = [c_int, POINTER(Source), POINTER(Options), POINTER(Fold), POINTER(Stats)]

@@ -365,6 +365,28 @@ struct NoPost {
     __device__ float operator()(float v) const { return v; }
 };
 
+// A chain's point stages on a sweep's output values, in order (Lookup and
+// Reclass as internal/curve/scalar.go:39-114 through arith.hpp), so a
+// stencil followed by point stages writes the chain's value straight from
+// registers. The tables are kernel arguments, read as uniform values.
+struct ArgPost {
+    const PostArgs *a;
+    __device__ float operator()(float v) const
+    {
+        for (int i = 0; i < a->n; ++i) {
+            const float *t = a->taps + a->off[i];
+            if (a->op[i] == STRATA_OP_LOOKUP) {
+                const int n = a->cnt[i] / 2;
+                v = lookup_of(v, t, t + n, n);
+            } else {
+                const int nb = (a->cnt[i] - 1) / 2;
+                v = reclass_of(v, t, nb, t + nb);
+            }
+        }
+        return v;
+    }
+};
+
 // MODE (wave-uniform, chosen per block or unit):
 //   kSweepChecked   every cell loaded and stored on its own, columns clamped
 //                   into the buffer and stores kept inside the region;
@@ -637,6 +659,34 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
             sweep<OP, kSweepPairs>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
         else
             sweep<OP, kSweepChecked>(a, rg, s, d0, d1, xb, y0, y1, threadIdx.x);
+    }
+}
+
+// horn_sweep with a chain's point stages applied to every stored value
+// (single-output ops).
+template <int OP, bool UNALIGNED>
+__global__ __launch_bounds__(kSweepThreads) void horn_sweep_post(HornArgs a, Region rg, SrcView s, DstView d0,
+                                                                 PostArgs pa)
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
+    const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
+    const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
+    const bool inside = xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
+    const ArgPost post{&pa};
+    if constexpr (UNALIGNED) {
+        if (inside)
+            sweep<OP, kSweepAnyParity>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
+    } else {
+        if (inside && a.aligned)
+            sweep<OP, kSweepPairs>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
     }
 }
 
@@ -1921,6 +1971,17 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
         e0.p += sb * a.rows * e0.stride;
         e1.p += sb * a.rows * e1.stride;
         const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
+        if constexpr (OP != STRATA_OP_GRADIENT) {
+            if (P.post.n > 0) {
+                if (aligned)
+                    hipLaunchKernelGGL((horn_sweep_post<OP, false>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0,
+                                       P.post);
+                else
+                    hipLaunchKernelGGL((horn_sweep_post<OP, true>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0,
+                                       P.post);
+                continue;
+            }
+        }
         if (aligned)
             hipLaunchKernelGGL((horn_sweep<OP, false>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0, e1);
         else

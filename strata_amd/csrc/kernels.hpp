@@ -61,14 +61,31 @@ struct FanArgs {
 // Internal op id of a FanArgs launch (never crosses the C ABI).
 constexpr int32_t kOpHornFan = 100;
 
+// Point stages (Lookup, Reclass) applied in order to every value a
+// radius-1 Horn sweep stores: the trailing stages of a chain, fused into
+// the stencil's store (a chain of exec.Pipeline, internal/exec/
+// pipeline.go:187, run as one kernel). Stage i is op[i] with its table of
+// cnt[i] floats at taps[off[i]].
+constexpr int kMaxPost = 4;
+constexpr int kPostTaps = 256;
+struct PostArgs {
+    int32_t n;
+    int32_t op[kMaxPost];
+    int32_t off[kMaxPost];
+    int32_t cnt[kMaxPost];
+    float taps[kPostTaps];
+};
+
 // Resolved per-call constants: k[] as the C ABI's strata_kernel, taps
 // copied by value into the kernel arguments (read through the scalar
-// cache as uniform values); fan for kOpHornFan.
+// cache as uniform values); fan for kOpHornFan; post for a radius-1 Horn
+// op whose store takes a chain's point stages.
 struct Params {
     int32_t op, radius, flags, ntaps;
     float k[16];
     float taps[kMaxTaps];
     FanArgs fan;
+    PostArgs post;
 };
 
 // Launch-shape knobs (results never depend on them).

@@ -145,14 +145,27 @@ struct Group {
     int out[kMaxOutputs] = {};
 };
 
+// A group of a chain's stages: a stencil stage (or, leading the chain, a
+// point stage) and the point stages after it. A radius-1 Horn stage takes
+// them into its store (P.post); otherwise they run over the group's values
+// in place. grow is how far past the output later groups read the group's
+// values.
+struct ChainGroup {
+    Params P;
+    int radius = 0, grow = 0;
+    std::vector<Params> inplace;
+};
+
 // A call's device work: its data launches, and each output's radius, which
 // is the width of its edge ring and the erosion of its validity. r is the
 // call's radius (how far its reads reach past an output cell) and rmin the
-// narrowest ring (internal/exec/tile.go:136-184).
+// narrowest ring (internal/exec/tile.go:136-184). A chain's groups run in
+// order instead of the launches (one output).
 struct Job {
     int nout = 0, r = 0, rmin = 0;
     int radius[kMaxDst] = {};
     std::vector<Group> groups;
+    std::vector<ChainGroup> chain;
 };
 
 Job make_job(const strata_kernel *k)
@@ -234,6 +247,70 @@ Job make_fan_job(const strata_kernel *parts, int nparts)
         if (fan.nout == kMaxOutputs) flush();
     }
     flush();
+    return J;
+}
+
+constexpr int kMaxChain = 8;  // stages of one chain
+
+bool is_point(const strata_kernel &k) { return k.op == STRATA_OP_LOOKUP || k.op == STRATA_OP_RECLASS; }
+
+// exec.NewPipeline(1, stages, [last]) with stage i reading stage i-1's
+// value (internal/exec/pipeline.go:166-187): one input, one output, every
+// stage a kernel of one input and one output. The chain's reach is the sum
+// of the stage radii (radii add along a chain), at most the library's 8.
+int check_chain(const strata_kernel *stages, int n)
+{
+    if (!stages || n < 1) return fail(STRATA_EINVAL, "engine: pipeline has no stages");
+    if (n > kMaxChain) return fail(STRATA_EINVAL, "engine: %d stages exceed the library's limit of %d", n, kMaxChain);
+    int reach = 0;
+    for (int i = 0; i < n; ++i) {
+        if (n_outputs(&stages[i]) != 1)
+            return fail(STRATA_EINVAL, "engine: stage %d has %d outputs; a chain stage has one", i,
+                        n_outputs(&stages[i]));
+        int rc = check_kernel(&stages[i], 1, 1);
+        if (rc) return rc;
+        reach += stages[i].radius;
+    }
+    if (reach > kMaxRadius)
+        return fail(STRATA_EINVAL, "engine: the chain reaches %d cells; the library's limit is %d", reach, kMaxRadius);
+    return STRATA_OK;
+}
+
+// The chain's job: its output's ring and validity erosion are the chain's
+// reach, and so is its radius (internal/exec/pipeline.go:32-41, 83-100).
+Job make_chain_job(const strata_kernel *stages, int n)
+{
+    Job J;
+    J.nout = 1;
+    for (int i = 0; i < n; ++i) {
+        const strata_kernel &k = stages[i];
+        J.r += k.radius;
+        if (is_point(k) && !J.chain.empty()) {
+            ChainGroup &g = J.chain.back();
+            PostArgs &pa = g.P.post;
+            const bool horn = (g.P.op == STRATA_OP_SLOPE || g.P.op == STRATA_OP_ASPECT ||
+                               g.P.op == STRATA_OP_HILLSHADE || g.P.op == STRATA_OP_CURVATURE) &&
+                              !(g.P.flags & STRATA_FLAG_FIT) && g.inplace.empty();
+            const int used = pa.n ? pa.off[pa.n - 1] + pa.cnt[pa.n - 1] : 0;
+            if (horn && pa.n < kMaxPost && used + k.ntaps <= kPostTaps) {
+                pa.op[pa.n] = k.op;
+                pa.off[pa.n] = used;
+                pa.cnt[pa.n] = k.ntaps;
+                memcpy(pa.taps + used, k.taps, sizeof(float) * (size_t)k.ntaps);
+                ++pa.n;
+            } else {
+                g.inplace.push_back(make_params(&k));
+            }
+            continue;
+        }
+        ChainGroup g;
+        g.P = make_params(&k);
+        g.radius = k.radius;
+        J.chain.push_back(g);
+    }
+    for (size_t g = J.chain.size() - 1; g > 0; --g) J.chain[g - 1].grow = J.chain[g].grow + J.chain[g].radius;
+    J.rmin = J.r;
+    J.radius[0] = J.r;
     return J;
 }
 
@@ -328,12 +405,73 @@ hipEvent_t timing_event(hipStream_t st)
     return e;
 }
 
+// A chain's groups over the output rect rc: every group but the last
+// writes its values over rc grown by its `grow` (clipped to the raster)
+// into a buffer from the library's pool, which the next group reads as its
+// input; the last writes the output. Each stage computes its own edge ring
+// (NaN), which the stages after it carry through, so the cells within the
+// chain's reach of the raster's edge hold NaN as the separate calls leave
+// them (internal/exec/pipeline.go:102-114).
+int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
+{
+    const int64_t W = op.W, H = op.H;
+    SrcView s = srcview(*op.src, op.sx, op.sy, W, H);
+    void *prev = nullptr;
+    struct Free {
+        void *&p;
+        hipStream_t st;
+        ~Free()
+        {
+            if (p) (void)hipFreeAsync(p, st);
+        }
+    } guard{prev, st};
+    for (size_t g = 0; g < J.chain.size(); ++g) {
+        const ChainGroup &G = J.chain[g];
+        const bool last = g + 1 == J.chain.size();
+        Rect out = rc;
+        DstView d;
+        void *buf = nullptr;
+        if (last) {
+            d.p = op.dst[0]->data + (rc.y0 - op.dy[0]) * op.dst[0]->stride + (rc.x0 - op.dx[0]);
+            d.stride = op.dst[0]->stride;
+        } else {
+            out.x0 = std::max<int64_t>(0, rc.x0 - G.grow);
+            out.y0 = std::max<int64_t>(0, rc.y0 - G.grow);
+            out.x1 = std::min<int64_t>(W, rc.x1 + G.grow);
+            out.y1 = std::min<int64_t>(H, rc.y1 + G.grow);
+            const int64_t bs = (out.x1 - out.x0 + 63) & ~(int64_t)63;
+            HIP_TRY(scratch_alloc_async(&buf, (size_t)((out.y1 - out.y0) * bs) * 4, st), "a chain's buffer");
+            d.p = (float *)buf;
+            d.stride = bs;
+        }
+        const Region rg{W, H, out.x0, out.y0, out.x1 - out.x0, out.y1 - out.y0};
+        HIP_TRY(launch_data(G.P, rg, s, &d, 1, st), "launching a chain stage");
+        for (const Params &Q : G.inplace) {
+            const SrcView v{d.p, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
+            HIP_TRY(launch_data(Q, rg, v, &d, 1, st), "launching a chain's point stage");
+        }
+        if (prev) HIP_TRY(hipFreeAsync(prev, st), "freeing a chain's buffer");
+        prev = buf;
+        if (!last) s = SrcView{(const float *)buf, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
+    }
+    return STRATA_OK;
+}
+
 int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
 {
     Region rg{op.W, op.H, rc.x0, rc.y0, rc.x1 - rc.x0, rc.y1 - rc.y0};
     if (rg.w <= 0 || rg.h <= 0) return STRATA_OK;
     const SrcView s = srcview(*op.src, op.sx, op.sy, op.W, op.H);
     const bool timed = g_timing.on;
+    if (!J.chain.empty()) {
+        hipEvent_t e0 = timed ? timing_event(st) : nullptr;
+        const int e = run_chain_data(J, op, rc, st);
+        if (e) return e;
+        if (timed) {
+            std::lock_guard<std::mutex> lk(g_timing_mu);
+            g_timing.data.emplace_back(e0, timing_event(st));
+        }
+    }
     for (const Group &g : J.groups) {
         DstView d[kMaxOutputs];
         for (int i = 0; i < g.nout; ++i) {
@@ -1832,6 +1970,30 @@ int strata_process_fanout_chunked(const strata_kernel *parts, int nparts, const 
     int rc = check_parts(parts, nparts);
     if (rc) return rc;
     return process_chunked(make_fan_job(parts, nparts), dst, nparts, src, 1, opts, stats);
+}
+
+int strata_plan_stats_chain(const strata_kernel *stages, int nstages, int64_t width, int64_t height,
+                            const strata_options *opts, int chunked, strata_stats *stats)
+{
+    int rc = check_chain(stages, nstages);
+    if (rc) return rc;
+    return plan_stats(make_chain_job(stages, nstages), 1, width, height, opts, chunked, stats);
+}
+
+int strata_process_chain(const strata_kernel *stages, int nstages, const strata_raster *dst, const strata_raster *src,
+                         const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_chain(stages, nstages);
+    if (rc) return rc;
+    return process(make_chain_job(stages, nstages), dst, 1, src, 1, opts, stats);
+}
+
+int strata_process_chain_chunked(const strata_kernel *stages, int nstages, const strata_sink *dst,
+                                 const strata_source *src, const strata_options *opts, strata_stats *stats)
+{
+    int rc = check_chain(stages, nstages);
+    if (rc) return rc;
+    return process_chunked(make_chain_job(stages, nstages), dst, 1, src, 1, opts, stats);
 }
 
 } // extern "C"

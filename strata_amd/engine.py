@@ -191,6 +191,42 @@ class FanOut:
         return arr, keep
 
 
+class Pipeline:
+    """exec.NewPipeline(1, stages, [last]) over a chain: stage i reads stage
+    i-1's value, stage 0 the pipeline's one input, and the last stage's
+    value is its output (internal/exec/pipeline.go:166-187), the pipeline
+    graph.Slope -> graph.Lookup -> graph.Reclass lowers to
+    (graph/graph.go:250-287). Its radius is the sum of the stage radii
+    (radii add along a chain), which is also its output's edge ring and
+    validity erosion; the output is what the stages write as separate
+    calls. The library takes the point stages after a radius-1 terrain
+    stage into that stage's store."""
+
+    def __init__(self, stages: Sequence[KernelDesc], name: str = "engine.Pipeline"):
+        if not stages:
+            raise EnginePanic("engine: pipeline has no stages")
+        for i, k in enumerate(stages):
+            if k is None:
+                raise EnginePanic(f"engine: stage {i} has a nil kernel")
+            if k.nin != 1 or k.nout != 1:
+                raise EnginePanic(f"engine: stage {i} takes {k.nin} inputs and {k.nout} outputs; a chain stage "
+                                  "takes one of each")
+        self.stages = list(stages)
+        self.radius = sum(k.radius for k in self.stages)
+        self.nin = 1
+        self.nout = 1
+        self.name = name
+
+    def to_c(self):
+        arr = (_lib.Kernel * len(self.stages))()
+        keep = []
+        for i, k in enumerate(self.stages):
+            kc, taps = k.to_c()
+            arr[i] = kc
+            keep.append(taps)
+        return arr, keep
+
+
 # -- overlap checks (internal/overlap/overlap.go) ------------------------------
 
 def _span(r: Float32Raster) -> int:
@@ -352,6 +388,8 @@ def ProcessN(ctx: Optional[Context], dst: List[Float32Raster], src: List[Float32
     co = _c_options(opts, ctx if (ctx is not None and not async_) else None, _stream_handle(dst + src), async_)
     if isinstance(k, FanOut):
         rc = L.strata_process_fanout(kc, len(k.parts), cd, cs, ctypes.byref(co), ctypes.byref(st))
+    elif isinstance(k, Pipeline):
+        rc = L.strata_process_chain(kc, len(k.stages), cd, cs, ctypes.byref(co), ctypes.byref(st))
     else:
         rc = L.strata_process(ctypes.byref(kc), cd, len(dst), cs, len(src), ctypes.byref(co), ctypes.byref(st))
     del taps
@@ -374,6 +412,9 @@ def PlanStats(k: KernelDesc, width: int, height: int, opts: Optional[Options] = 
     if isinstance(k, FanOut):
         rc = _lib.lib().strata_plan_stats_fanout(kc, len(k.parts), width, height, ctypes.byref(co),
                                                  1 if chunked else 0, ctypes.byref(st))
+    elif isinstance(k, Pipeline):
+        rc = _lib.lib().strata_plan_stats_chain(kc, len(k.stages), width, height, ctypes.byref(co),
+                                                1 if chunked else 0, ctypes.byref(st))
     else:
         rc = _lib.lib().strata_plan_stats(ctypes.byref(kc), k.nout, k.nin, width, height, ctypes.byref(co),
                                           1 if chunked else 0, ctypes.byref(st))
@@ -1001,6 +1042,9 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
     if isinstance(k, FanOut):
         rc = _lib.lib().strata_process_fanout_chunked(kc, len(k.parts), cdst, csrc, ctypes.byref(co),
                                                       ctypes.byref(st))
+    elif isinstance(k, Pipeline):
+        rc = _lib.lib().strata_process_chain_chunked(kc, len(k.stages), cdst, csrc, ctypes.byref(co),
+                                                     ctypes.byref(st))
     else:
         rc = _lib.lib().strata_process_chunked(ctypes.byref(kc), cdst, len(dst), csrc, len(src),
                                                ctypes.byref(co), ctypes.byref(st))

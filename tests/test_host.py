@@ -302,3 +302,35 @@ def test_raster_model():
     assert Float32Raster(np.zeros(3, np.float32), 2, 2).Validate().startswith("raster: data has 3 cells")
     with pytest.raises(EnginePanic.__mro__[1]):
         r.Window(3, 3, 3, 3)
+
+
+CHAIN_STACKS = [
+    [terrain.slopeOp(terrain.SlopeOptions(CellSize=30)), transfer.newLookupKernel("t", [0, 5], [1, 2]),
+     transfer.newReclassKernel("t", [1.5], [1, 2])],
+    [focal.newBox(focal.BoxOptions(2), "mean"), terrain.slopeOp(terrain.SlopeOptions(CellSize=30))],
+    [focal.newBox(focal.BoxOptions(1), "max"), focal.newBox(focal.BoxOptions(2), "mean"),
+     terrain.curvatureOp(terrain.CurvatureOptions(CellSize=10))],
+]
+
+
+@pytest.mark.parametrize("si", range(len(CHAIN_STACKS)))
+@pytest.mark.parametrize("W,H", [(1, 1), (5, 3), (17, 5), (300, 200), (70000, 3)])
+@pytest.mark.parametrize("tw,th", [(0, 0), (7, 5), (64, 1), (256, 256)])
+@pytest.mark.parametrize("chunked", [False, True])
+def test_chain_plan_stats_match_reference_loops(si, W, H, tw, th, chunked):
+    """A chain is one kernel whose radius is the sum of its stages' radii
+    (internal/exec/pipeline.go:32-41): its Stats are that kernel's."""
+    pipe = engine.Pipeline(CHAIN_STACKS[si])
+    assert pipe.radius == sum(k.radius for k in CHAIN_STACKS[si])
+    got = PlanStats(pipe, W, H, Options(TileWidth=tw, TileHeight=th), chunked=chunked)
+    assert got == ref_stats(pipe, W, H, tw, th, chunked)
+
+
+def test_chain_checks():
+    with pytest.raises(EnginePanic):
+        engine.Pipeline([])
+    with pytest.raises(EnginePanic):
+        engine.Pipeline([terrain.gradientOp(terrain.GradientOptions(CellSize=1))])
+    too_far = engine.Pipeline([focal.newBox(focal.BoxOptions(5), "mean"), focal.newBox(focal.BoxOptions(4), "max")])
+    with pytest.raises(EnginePanic, match="reaches 9"):
+        PlanStats(too_far, 100, 100)
