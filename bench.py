@@ -76,6 +76,8 @@ def parse():
                     help="timed runs of each reduction over the DEM (0: skip)")
     ap.add_argument("--raw-size", type=int, default=8192,
                     help="side of the file-to-file SlopeChunked run (raw float32 files; 0: skip)")
+    ap.add_argument("--no-paths", dest="paths", action="store_false",
+                    help="skip the cancellable, chain and host-memory Slope runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of the DEM (for HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -148,11 +150,16 @@ def main() -> int:
     value = cells * args.steps / el / 1e9
     kern_ms = timing.data_ms / max(1, timing.data_launches)
     vkern_ms = timing.validity_ms / max(1, timing.validity_launches)
-    data_bytes = 8 * cells  # 4 B read + 4 B written per cell
+    # The dominant kernel: the sweep, which also writes the validity when no
+    # validity launch of its own was recorded (4 B read + 4 B written per
+    # cell, plus 1/8 B of mask in and out when it writes the validity).
+    fused = timing.validity_launches == 0
+    data_bytes = (8.25 if fused else 8) * cells
+    kernel_name = "horn_sweep_valid<SLOPE> (data and validity)" if fused else "horn_sweep<SLOPE> (data kernel)"
     achieved = data_bytes / (kern_ms * 1e-3) / 1e9
     op_bytes = 8.25 * cells  # plus 1/8 B mask in and out
-    log(f"bench: Slope {ms_step:.3f} ms/step, {value:.1f} Gcells/s; data kernel {kern_ms:.3f} ms "
-        f"({achieved:.0f} GB/s), validity kernel {vkern_ms:.3f} ms")
+    log(f"bench: Slope {ms_step:.3f} ms/step, {value:.1f} Gcells/s; {kernel_name} {kern_ms:.3f} ms "
+        f"({achieved:.0f} GB/s), separate validity kernel {vkern_ms:.3f} ms")
 
     # Statistics of the configuration, in the reference's closed form.
     st = Stats()
@@ -220,6 +227,10 @@ def main() -> int:
                                           "speedup": round(sep / fused, 3)}
         del out2, extra
 
+    paths = None
+    if args.paths:
+        paths = paths_bench(args, dem, out, src, sink, slope, N, dev, log)
+
     traffic = None
     if not args.no_traffic:
         traffic = pmc_traffic(args, N, log)
@@ -278,7 +289,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes_per_launch"] if traffic and "bytes_per_launch" in traffic else None,
                      "traffic_detail": traffic,
-                     "kernel": "horn_sweep<SLOPE> (data kernel)", "bytes_per_launch": data_bytes,
+                     "kernel": kernel_name, "bytes_per_launch": data_bytes,
                      "kernel_ms": round(kern_ms, 4), "launches": timing.data_launches,
                      # engine.Stats of one op (the reference's byte accounting, in closed
                      # form) beside the bytes the device moved (counters, data kernel).
@@ -299,6 +310,7 @@ def main() -> int:
         "focal": focal_res,
         "reduce": reduce_res,
         "file_to_file": raw_res,
+        "paths": paths,
     }
     if serving and serving["levels"]:
         at = min(serving["levels"], key=lambda lv: abs(lv["qps"] - 1000.0))
@@ -307,6 +319,111 @@ def main() -> int:
                                    "arrival to completion"}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def paths_bench(args, dem, out, src, sink, slope, N, dev, log):
+    """The other ways a caller runs the headline op over the same DEM:
+
+    - cancellable: ProcessN with a live context (plan units claimed in
+      order, two in flight) against the same call with none (one launch);
+    - chain: Slope -> Lookup -> Reclass (the serving chain's tables) as one
+      exec.Pipeline call, SlopeChunked-style over the resident DEM, against
+      the three calls it replaces, with the outputs compared bit for bit;
+    - host memory: the DEM and the output in host (pageable) memory, as a
+      Go caller hands them over, through ProcessN (staged whole) and through
+      ProcessChunked with TileHeight 256 (tiles read and written through the
+      host callbacks), in Gcells/s and in GB/s over PCIe; the output is
+      compared with the device-resident result bit for bit."""
+    import numpy as np
+    import torch
+
+    from strata_amd import transfer
+    from strata_amd.engine import Context, MemorySink, MemorySource, Options, Pipeline, ProcessChunked, ProcessN
+    from strata_amd.raster import Float32Raster
+
+    cells = N * N
+    res = {}
+    steps = max(5, args.steps // 2)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    plain_s = timed(lambda: ProcessN(None, [out], [dem], slope, Options()), steps)
+    live_s = timed(lambda: ProcessN(Context(), [out], [dem], slope, Options()), steps)
+    res["cancellable"] = {"plain_ms": round(plain_s * 1e3, 4), "cancellable_ms": round(live_s * 1e3, 4),
+                          "overhead": round(live_s / plain_s - 1, 4),
+                          "what": "ProcessN Slope over the resident DEM, synchronous calls, with and without a "
+                                  "live context"}
+    log(f"bench: cancellable {res['cancellable']}")
+
+    lookup = transfer.newLookupKernel("risk", [0, 5, 10, 20, 30, 40], [1, 1.1, 1.3, 1.9, 3, 4.5])
+    reclass = transfer.newReclassKernel("risk", [1.2, 1.6, 2.2, 3], [1, 2, 3, 4, 5])
+    pipe = Pipeline([slope, lookup, reclass])
+    tmp = Float32Raster(torch.empty(cells, dtype=torch.float32, device=dev), N, N, N,
+                        torch.zeros_like(dem.Valid), 0)
+    eopts = Options(TileHeight=args.tile_height)
+    tsink = MemorySink(tmp)
+
+    def separate():
+        ProcessChunked(None, [sink], [src], slope, eopts, async_=True)
+        ProcessChunked(None, [tsink], [MemorySource(out)], lookup, eopts, async_=True)
+        ProcessChunked(None, [sink], [MemorySource(tmp)], reclass, eopts, async_=True)
+
+    sep_s = timed(separate, steps)
+    want = (out.Data.clone(), out.Valid.clone())
+    chain_s = timed(lambda: ProcessChunked(None, [sink], [src], pipe, eopts, async_=True), steps)
+    same = bool(torch.equal(out.Data.view(torch.int32), want[0].view(torch.int32)) and
+                torch.equal(out.Valid, want[1]))
+    res["chain"] = {"chain_ms": round(chain_s * 1e3, 4), "gcells_s": round(cells / chain_s / 1e9, 2),
+                    "three_calls_ms": round(sep_s * 1e3, 4), "speedup": round(sep_s / chain_s, 3),
+                    "identical": same,
+                    "what": f"Slope -> Lookup -> Reclass over the resident {N}x{N} DEM, TileHeight "
+                            f"{args.tile_height}, one pipeline call against three calls"}
+    log(f"bench: chain {res['chain']}")
+    del tmp, tsink, want
+
+    # Host memory. The device result of Slope is the reference bits.
+    ProcessChunked(None, [sink], [src], slope, eopts)
+    want_d = out.Data.cpu().numpy()
+    want_v = out.Valid.cpu().numpy().view(np.uint64)
+    hsrc = Float32Raster(dem.Data.cpu().numpy(), N, N, N, dem.Valid.cpu().numpy().view(np.uint64).copy(), 0)
+    hout = Float32Raster(np.zeros(cells, np.float32), N, N, N, np.zeros(len(want_v), np.uint64), 0)
+    mask_b = 8 * len(want_v)
+
+    def same_host():
+        return bool(np.array_equal(hout.Data.view(np.uint32), want_d.view(np.uint32)) and
+                    np.array_equal(hout.Valid, want_v))
+
+    reps = 3
+    h_plain = timed(lambda: ProcessN(None, [hout], [hsrc], slope, Options()), reps)
+    ok_plain = same_host()
+    hout.Data[:] = 0
+    hout.Valid[:] = 0
+    h_chunk = timed(lambda: ProcessChunked(None, [MemorySink(hout)], [MemorySource(hsrc)], slope, eopts), reps)
+    ok_chunk = same_host()
+    # Bytes over PCIe: ProcessN moves the source and its mask in, the output's
+    # mask in and out (its words are shared with cells outside it in general)
+    # and the output's cells out; the chunked tiles move each tile's window
+    # with its halo rows in and the tile out, with their masks.
+    plain_b = 4 * cells * 2 + 3 * mask_b
+    chunk_b = 4 * cells * 2 + 2 * mask_b
+    res["host_memory"] = {
+        "plain": {"ms": round(h_plain * 1e3, 2), "gcells_s": round(cells / h_plain / 1e9, 3),
+                  "pcie_gb_s": round(plain_b / h_plain / 1e9, 2), "identical": ok_plain},
+        "chunked": {"ms": round(h_chunk * 1e3, 2), "gcells_s": round(cells / h_chunk / 1e9, 3),
+                    "pcie_gb_s": round(chunk_b / h_chunk / 1e9, 2), "identical": ok_chunk,
+                    "tile_height": args.tile_height},
+        "what": f"terrain.Slope {N}x{N} with the DEM and the output in host (pageable) memory: ProcessN "
+                f"(whole rasters staged through HBM) and ProcessChunked (MemorySource/MemorySink tiles through "
+                f"the host callbacks)"}
+    log(f"bench: host memory {res['host_memory']}")
+    return res
 
 
 def raw_bench(args, dev, log):
@@ -490,7 +607,7 @@ def pmc_child(args) -> int:
 
 
 def pmc_traffic(args, N, log):
-    """HBM bytes per launch of the Slope data kernel from two rocprofv3
+    """HBM bytes per launch of the Slope kernel from two rocprofv3
     counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
     scaled by what the same counter reads for a plain device copy of the
     DEM (4·N² bytes read and written) in the same run."""
@@ -521,7 +638,9 @@ def pmc_traffic(args, N, log):
                 for row in csv.DictReader(f):
                     name = row.get("Kernel_Name", "")
                     v = float(row["Counter_Value"])
-                    if "horn_sweep<2, false>" in name or "horn_sweep<2>" in name:  # the aligned Slope sweep
+                    # The aligned Slope sweep, with or without the validity waves.
+                    if any(k in name for k in ("horn_sweep<2, false>", "horn_sweep<2>", "horn_sweep_valid<2, false>",
+                                               "horn_sweep_valid<2>")):
                         slope_vals.append(v)
                     elif "copyBuffer" in name:
                         copy_vals.append(v)
@@ -536,7 +655,8 @@ def pmc_traffic(args, N, log):
            "fetch_kib_raw": round(got["FETCH_SIZE"][0], 1), "write_kib_raw": round(got["WRITE_SIZE"][0], 1),
            "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
            "calibration": f"device copy of the {N}x{N} DEM ({copy_kib:.0f} KiB read and written), same run",
-           "algorithmic_bytes": 8 * N * N}
+           "algorithmic_bytes": 8.25 * N * N, "algorithmic_note": "4 B in and out per cell and 1/8 B of mask "
+                                                                   "in and out (the sweep writes the validity)"}
     log(f"bench: traffic {res['bytes_per_launch'] / 1e9:.3f} GB/launch (read {fetch / 1e9:.3f}, write "
         f"{write / 1e9:.3f}; factors {ff:.3f}/{wf:.3f})")
     return res

@@ -1585,6 +1585,46 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
     validity_strip<R>(rg, src, has_src, dst, wx, y0, y1);
 }
 
+// A radius-1 Horn op with one masked output, data and validity in one
+// launch: blocks [0, data_blocks) sweep as horn_sweep does, the blocks after
+// them run validity_strip waves (four per block), so the validity's short
+// work fills the sweep's tail instead of a launch of its own.
+template <int OP, bool UNALIGNED>
+__global__ __launch_bounds__(kSweepThreads) void horn_sweep_valid(HornArgs a, Region rg, SrcView s, DstView d0,
+                                                                  uint32_t data_blocks, MaskView vsrc, int has_src,
+                                                                  MaskView vdst, uint32_t waves_x)
+{
+    const uint32_t b = blockIdx.x;
+    if (b >= data_blocks) {
+        const uint32_t wv = (b - data_blocks) * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const uint32_t wx = wv % waves_x, strip = wv / waves_x;
+        const int64_t y0 = rg.oy + (int64_t)strip * kVsRows;
+        const int64_t y1 = min(y0 + (int64_t)kVsRows, rg.oy + rg.h);
+        if (y0 >= y1) return;  // past the last strip
+        validity_strip<1>(rg, vsrc, has_src, vdst, wx, y0, y1);
+        return;
+    }
+    constexpr uint32_t kXcds = 8;
+    const uint32_t nb = data_blocks;
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
+    const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
+    const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
+    const bool inside = xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
+    if constexpr (UNALIGNED) {
+        if (inside)
+            sweep<OP, kSweepAnyParity>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x);
+    } else {
+        if (inside && a.aligned)
+            sweep<OP, kSweepPairs>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x);
+        else
+            sweep<OP, kSweepChecked>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x);
+    }
+}
+
 // validity_strip's rule for a narrow region, whose row (its output words
 // and four helper lanes) fits in G <= 32 lanes: the wave's lanes form
 // 64 / G groups of G lanes, group g taking the region's rows
@@ -2173,6 +2213,56 @@ hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, cons
     case STRATA_OP_RECLASS: return run_point<STRATA_OP_RECLASS>(P, rg, s, d, st);
     }
     return hipErrorInvalidValue;
+}
+
+template <int OP>
+static hipError_t run_horn_valid(const Params &P, const Region &rg, const SrcView &s, const DstView &d,
+                                 const MaskView &vsrc, int has_src, const MaskView &vdst, hipStream_t st, bool *fused)
+{
+    HornArgs a;
+    for (int i = 0; i < 8; ++i) a.k[i] = P.k[i];
+    a.flags = P.flags;
+    auto al8 = [](const void *p) { return ((uintptr_t)p & 7) == 0; };
+    const bool aligned = al8(s.p) && (s.stride & 1) == 0 && ((rg.ox - s.sx) & 1) == 0 && al8(d.p) &&
+                         (d.stride & 1) == 0;
+    a.aligned = aligned ? 1 : 0;
+    a.rows = sweep_rows();
+    const int64_t gx = cdiv(rg.w, kSweepCols);
+    a.blocks_x = (uint32_t)gx;
+    const int64_t data_blocks = gx * cdiv(rg.h, a.rows);
+    const int64_t waves_x = cdiv(rg.w / 32 + 2, kVsOut);
+    const int64_t vblocks = cdiv(waves_x * cdiv(rg.h, kVsRows), 4);
+    // One launch of at most 2^31 blocks; larger regions take the two kernels.
+    if (data_blocks + vblocks > ((int64_t)1 << 31) - 1) {
+        *fused = false;
+        return hipSuccess;
+    }
+    const dim3 grid((unsigned)(data_blocks + vblocks));
+    if (aligned)
+        hipLaunchKernelGGL((horn_sweep_valid<OP, false>), grid, dim3(kSweepThreads), 0, st, a, rg, s, d,
+                           (uint32_t)data_blocks, vsrc, has_src, vdst, (uint32_t)waves_x);
+    else
+        hipLaunchKernelGGL((horn_sweep_valid<OP, true>), grid, dim3(kSweepThreads), 0, st, a, rg, s, d,
+                           (uint32_t)data_blocks, vsrc, has_src, vdst, (uint32_t)waves_x);
+    *fused = true;
+    return hipGetLastError();
+}
+
+hipError_t launch_data_valid(const Params &P, const Region &rg, const SrcView &s, const DstView &d,
+                             const MaskView &vsrc, int has_src, const MaskView &vdst, hipStream_t st, bool *fused)
+{
+    *fused = false;
+    if (rg.w <= 0 || rg.h <= 0) return hipSuccess;
+    if ((P.flags & STRATA_FLAG_FIT) || P.radius != 1 || P.post.n > 0) return hipSuccess;
+    switch (P.op) {
+    case STRATA_OP_SLOPE: return run_horn_valid<STRATA_OP_SLOPE>(P, rg, s, d, vsrc, has_src, vdst, st, fused);
+    case STRATA_OP_ASPECT: return run_horn_valid<STRATA_OP_ASPECT>(P, rg, s, d, vsrc, has_src, vdst, st, fused);
+    case STRATA_OP_HILLSHADE:
+        return run_horn_valid<STRATA_OP_HILLSHADE>(P, rg, s, d, vsrc, has_src, vdst, st, fused);
+    case STRATA_OP_CURVATURE:
+        return run_horn_valid<STRATA_OP_CURVATURE>(P, rg, s, d, vsrc, has_src, vdst, st, fused);
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has_src, const MaskView &dst,

@@ -108,6 +108,14 @@ hipError_t scratch_alloc_async(void **p, size_t bytes, hipStream_t st);
 hipError_t launch_data(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
                        hipStream_t st);
 
+// Data and validity of a radius-1 Horn op (Slope, Aspect, Hillshade,
+// Curvature; no fit, no fused point stages) with one masked output in one
+// launch: the validity waves run as extra workgroups after the sweep's.
+// *fused tells whether it launched; when not, the caller launches
+// launch_data and launch_validity.
+hipError_t launch_data_valid(const Params &P, const Region &rg, const SrcView &s, const DstView &d,
+                             const MaskView &vsrc, int has_src, const MaskView &vdst, hipStream_t st, bool *fused);
+
 // Validity of the region's cells of one output: interior AND every cell of
 // the (2r+1)^2 box valid in the source (has_src), or interior alone.
 hipError_t launch_validity(int r, const Region &rg, const MaskView &src, int has_src, const MaskView &dst,

@@ -472,6 +472,8 @@ int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
             g_timing.data.emplace_back(e0, timing_event(st));
         }
     }
+    bool valid_done[kMaxDst] = {};
+    const bool has_src = op.src->valid != nullptr;
     for (const Group &g : J.groups) {
         DstView d[kMaxOutputs];
         for (int i = 0; i < g.nout; ++i) {
@@ -480,16 +482,26 @@ int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
             d[i].stride = op.dst[o]->stride;
         }
         hipEvent_t e0 = timed ? timing_event(st) : nullptr;
-        HIP_TRY(launch_data(g.P, rg, s, d, g.nout, st), "launching the data kernel");
+        bool fused = false;
+        if (g.nout == 1 && op.dst[g.out[0]]->valid) {
+            // A radius-1 Horn op writes its one output's validity in the
+            // same launch as its data.
+            const int o = g.out[0];
+            const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o], op.W, op.H);
+            const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy, op.W, op.H) : dm;
+            HIP_TRY(launch_data_valid(g.P, rg, s, d[0], sm, has_src ? 1 : 0, dm, st, &fused),
+                    "launching the data and validity kernel");
+            valid_done[o] = fused && J.radius[o] == 1;
+        }
+        if (!fused) HIP_TRY(launch_data(g.P, rg, s, d, g.nout, st), "launching the data kernel");
         if (timed) {
             std::lock_guard<std::mutex> lk(g_timing_mu);
             g_timing.data.emplace_back(e0, timing_event(st));
         }
     }
     for (int o = 0; o < op.nout; ++o) {
-        if (!op.dst[o]->valid) continue;
+        if (!op.dst[o]->valid || valid_done[o]) continue;
         const MaskView dm = mask_view(*op.dst[o], op.dx[o], op.dy[o], op.W, op.H);
-        const bool has_src = op.src->valid != nullptr;
         const MaskView sm = has_src ? mask_view(*op.src, op.sx, op.sy, op.W, op.H) : dm;
         hipEvent_t v0 = timed ? timing_event(st) : nullptr;
         HIP_TRY(launch_validity(J.radius[o], rg, sm, has_src ? 1 : 0, dm, st), "launching the validity kernel");

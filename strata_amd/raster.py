@@ -245,10 +245,18 @@ def pack_bits(valid: np.ndarray, nbits: Optional[int] = None) -> np.ndarray:
 
 
 def unpack_bits(m, off: int, n: int) -> np.ndarray:
-    """Bits [off, off+n) of a mask as a bool array."""
-    arr = m.detach().cpu().numpy().view(np.uint64) if _is_torch(m) else np.asarray(m, dtype=np.uint64)
-    bits = np.unpackbits(arr.view(np.uint8), bitorder="little").astype(bool)
-    return bits[off:off + n]
+    """Bits [off, off+n) of a mask as a bool array (only the words holding
+    them are read)."""
+    if n <= 0:
+        return np.zeros(0, dtype=bool)
+    w0, w1 = off >> 6, (off + n - 1) >> 6
+    if _is_torch(m):
+        arr = m[w0:w1 + 1].detach().cpu().numpy().view(np.uint64)
+    else:
+        arr = np.asarray(m, dtype=np.uint64)[w0:w1 + 1]
+    bits = np.unpackbits(np.ascontiguousarray(arr).view(np.uint8), bitorder="little").astype(bool)
+    s = off - (w0 << 6)
+    return bits[s:s + n]
 
 
 def valid_grid(r: Float32Raster) -> np.ndarray:
