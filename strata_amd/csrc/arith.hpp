@@ -172,33 +172,45 @@ __device__ __forceinline__ float zt_curvature(float z1, float z2, float z3, floa
 __device__ __forceinline__ float go_min(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 __device__ __forceinline__ float go_max(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 
-// internal/curve/scalar.go:39-56 (Reclass) and 84-114 (Lookup).
+// internal/curve/scalar.go:39-56 (Reclass) and 84-114 (Lookup). The scan
+// for the first knot above v runs over every knot with a uniform index and
+// keeps the first hit by selects: the tables are read with wave-uniform
+// indices only (scalar loads, no per-lane gathers and no divergent loop),
+// and the value is computed from the same knots in the same operations.
 __device__ __forceinline__ float reclass_of(float v, const float *breaks, int nb, const float *values)
 {
-    if (v != v) return v;
-    int k = nb;
-    for (int j = 0; j < nb; j++)
-        if (v < breaks[j]) {
-            k = j;
-            break;
-        }
-    return values[k];
+    float out = values[nb];
+    bool found = false;
+    for (int j = 0; j < nb; j++) {
+        const bool hit = !found && v < breaks[j];
+        out = hit ? values[j] : out;
+        found = found || hit;
+    }
+    return v != v ? v : out;
 }
 __device__ __forceinline__ float lookup_of(float v, const float *xs, const float *ys, int n)
 {
-    if (v != v) return v;
-    int k = n;
-    for (int j = 0; j < n; j++)
-        if (v < xs[j]) {
-            k = j;
-            break;
+    // state 2: no knot above v (k == n); 0: the first knot (k == 0); 1:
+    // between knots k-1 and k, which x0, y0, x1, y1 hold.
+    int state = 2;
+    float x0 = 0.0f, y0 = 0.0f, x1 = 1.0f, y1 = 0.0f, px = 0.0f, py = 0.0f;
+    for (int j = 0; j < n; j++) {
+        const float xj = xs[j], yj = ys[j];
+        const bool hit = state == 2 && v < xj;
+        if (hit) {
+            state = j == 0 ? 0 : 1;
+            x0 = px;
+            y0 = py;
+            x1 = xj;
+            y1 = yj;
         }
-    if (k == 0) return ys[0];
-    if (k == n) return ys[n - 1];
-    float x0 = xs[k - 1], y0 = ys[k - 1];
-    if (v == x0) return y0;
-    float t = (v - x0) / (xs[k] - x0);
-    return y0 + t * (ys[k] - y0);
+        px = xj;
+        py = yj;
+    }
+    const float t = (v - x0) / (x1 - x0);
+    const float mid = v == x0 ? y0 : y0 + t * (y1 - y0);
+    const float out = state == 0 ? ys[0] : (state == 2 ? ys[n - 1] : mid);
+    return v != v ? v : out;
 }
 
 } // namespace strata

@@ -370,11 +370,11 @@ struct NoPost {
 // stencil followed by point stages writes the chain's value straight from
 // registers. The tables are kernel arguments, read as uniform values.
 struct ArgPost {
-    const PostArgs *a;
+    const STRATA_GLOBAL PostArgs *a;
     __device__ float operator()(float v) const
     {
         for (int i = 0; i < a->n; ++i) {
-            const float *t = a->taps + a->off[i];
+            const float *t = (const float *)(a->taps + a->off[i]);
             if (a->op[i] == STRATA_OP_LOOKUP) {
                 const int n = a->cnt[i] / 2;
                 v = lookup_of(v, t, t + n, n);
@@ -429,6 +429,12 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     // offset, with no per-lane 64-bit address arithmetic.
     auto col = [&](int64_t c) { return 4u * (uint32_t)(min(max(c, s.lx0), s.lx1 - 1) - s.sx); };
     const uint32_t o0 = col(x), o1 = col(x + 1), oh = col(lane == 0 ? x - 1 : x + 2);
+    // Every lane issues the halo load of a row, lanes other than 0 and 63 at
+    // their own first cell (a cache hit): with the same loads on every lane
+    // and path, the row loaded ahead stays in flight while the current row
+    // is computed (a load under a divergent branch makes the wave wait for
+    // every outstanding access, stores included, before each row).
+    const uint32_t oho = halo_lane ? oh : o0;
     const uint32_t so = 4u * ((uint32_t)(xb - rg.ox) + off);  // store byte offset in a region row
     const bool ci0 = x >= 1 && x < rg.W - 1;
     const bool ci1 = x + 1 >= 1 && x + 1 < rg.W - 1;
@@ -458,24 +464,27 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
                 const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0);
                 c0 = v.x;
                 c1 = v.y;
+                h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oho);
             } else {
                 // Cells x - 1, x; lane 63 also x + 1, x + 2 (its right
-                // neighbour's pair).
+                // neighbour's pair), which the other lanes load as their own
+                // pair again.
                 const pair_t v = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0 - 4u);
                 c0 = v.x;
                 c1 = v.y;
-                if (lane == 63) {
-                    const pair_t e = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0 + 4u);
-                    h = e.x;
-                    h2 = e.y;
-                }
+                // The step is signed and added to the address: o0 is 0 for
+                // lane 0 at the buffer's first cell, and o0 - 4 in 32 bits
+                // would wrap to 4 GiB past the row.
+                const int32_t es = lane == 63 ? 4 : -4;
+                const pair_t e = *reinterpret_cast<const STRATA_GLOBAL pair_t *>(rp + o0 + es);
+                h = e.x;
+                h2 = e.y;
             }
         } else {
             c0 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o0);
             c1 = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + o1);
         }
-        if (MODE != kSweepAnyParity || !row.odd)
-            if (halo_lane) h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oh);
+        if constexpr (MODE != kSweepAnyParity) h = *reinterpret_cast<const STRATA_GLOBAL float *>(rp + oho);
         row.c0 = c0;
         row.c1 = c1;
         row.l = h;  // the halo values until finish() forms l and r
@@ -536,96 +545,103 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
     load(R[2], 1);
     finish(R[0]);
     finish(R[1]);
-    for (int32_t ub = 0; ub < nrows; ub += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int32_t uu = ub + u;
-            const int64_t yy = y0 + uu;
-            if (uu < nrows) {
-                Row &A = R[u & 3];
-                Row &B = R[(u + 1) & 3];
-                Row &C = R[(u + 2) & 3];
-                Row &D = R[(u + 3) & 3];
-                if (uu + 2 <= nrows) load(D, uu + 2);
-                finish(C);
-                const bool ri = uu >= ilo && uu < ihi;
-                const float nan = edge_nan();
-                if constexpr (OP == STRATA_OP_SURFACE) {
-                    // terrain.Surface (terrain/surface.go:52-102): every wanted
-                    // product from one gradient, each with its op's arithmetic.
-                    float gx0, gy0, gx1, gy1;
-                    horn_grad(a, A, B, C, gx0, gy0, gx1, gy1);
-                    auto put = [&](int o, float v0, float v1) {
-                        store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
-                    };
-                    const int f = a.flags;
-                    int o = 0;
-                    if (f & STRATA_SURF_DX) put(o++, gx0, gx1);
-                    if (f & STRATA_SURF_DY) put(o++, gy0, gy1);
-                    if (f & STRATA_SURF_SLOPE) {
-                        float v0, v1;
-                        slope_pair(gx0, gy0, gx1, gy1, a.k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
-                        put(o++, v0, v1);
-                    }
-                    if (f & STRATA_SURF_ASPECT) {
-                        float v0, v1;
-                        aspect_pair(gx0, gy0, gx1, gy1, a.k[3], (f & STRATA_FLAG_TRIG) != 0, v0, v1);
-                        put(o++, v0, v1);
-                    }
-                    if (f & STRATA_SURF_HILLSHADE) {
-                        float v0, v1;
-                        shade_pair(gx0, gy0, gx1, gy1, a.k[4], a.k[5], a.k[6], v0, v1);
-                        put(o++, v0, v1);
-                    }
-                } else if constexpr (OP == kOpHornFan) {
-                    // terrain.Features (terrain/features.go:68-117): each output
-                    // is its own op with its own constants. The unscaled Horn
-                    // differences are the same for every op; each scales them
-                    // by its own kx, ky, as the op alone does.
-                    const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
-                    const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
-                    const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
-                    const float dy0 = (gl + g1) + (g0 + g0);
-                    const float dy1 = (g0 + gr) + (g1 + g1);
-                    // Rolled: one copy of each op's code however many outputs
-                    // (unrolled, the kernel outgrows the instruction cache).
+    // One output row from ring slot u (rows A, B, C; D takes the next row).
+    auto step = [&](const int u, const int32_t uu) __attribute__((always_inline)) {
+        const int64_t yy = y0 + uu;
+        Row &A = R[u & 3];
+        Row &B = R[(u + 1) & 3];
+        Row &C = R[(u + 2) & 3];
+        Row &D = R[(u + 3) & 3];
+        // Always issued (past the strip it reloads the strip's last row, a
+        // cache hit), so no wait has to cover a load that may not exist.
+        load(D, min(uu + 2, nrows));
+        finish(C);
+        const bool ri = uu >= ilo && uu < ihi;
+        const float nan = edge_nan();
+        if constexpr (OP == STRATA_OP_SURFACE) {
+            // terrain.Surface (terrain/surface.go:52-102): every wanted
+            // product from one gradient, each with its op's arithmetic.
+            float gx0, gy0, gx1, gy1;
+            horn_grad(a, A, B, C, gx0, gy0, gx1, gy1);
+            auto put = [&](int o, float v0, float v1) {
+                store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
+            };
+            const int f = a.flags;
+            int o = 0;
+            if (f & STRATA_SURF_DX) put(o++, gx0, gx1);
+            if (f & STRATA_SURF_DY) put(o++, gy0, gy1);
+            if (f & STRATA_SURF_SLOPE) {
+                float v0, v1;
+                slope_pair(gx0, gy0, gx1, gy1, a.k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
+                put(o++, v0, v1);
+            }
+            if (f & STRATA_SURF_ASPECT) {
+                float v0, v1;
+                aspect_pair(gx0, gy0, gx1, gy1, a.k[3], (f & STRATA_FLAG_TRIG) != 0, v0, v1);
+                put(o++, v0, v1);
+            }
+            if (f & STRATA_SURF_HILLSHADE) {
+                float v0, v1;
+                shade_pair(gx0, gy0, gx1, gy1, a.k[4], a.k[5], a.k[6], v0, v1);
+                put(o++, v0, v1);
+            }
+        } else if constexpr (OP == kOpHornFan) {
+            // terrain.Features (terrain/features.go:68-117): each output
+            // is its own op with its own constants. The unscaled Horn
+            // differences are the same for every op; each scales them
+            // by its own kx, ky, as the op alone does.
+            const float dx0 = (A.hd0 + C.hd0) + (B.hd0 + B.hd0);
+            const float dx1 = (A.hd1 + C.hd1) + (B.hd1 + B.hd1);
+            const float gl = C.l - A.l, g0 = C.c0 - A.c0, g1 = C.c1 - A.c1, gr = C.r - A.r;
+            const float dy0 = (gl + g1) + (g0 + g0);
+            const float dy1 = (g0 + gr) + (g1 + g1);
+            // Rolled: one copy of each op's code however many outputs
+            // (unrolled, the kernel outgrows the instruction cache).
 #pragma unroll 1
-                    for (int o = 0; o < kMaxOutputs; ++o) {
-                        if (o < fan->n) {
-                            const float *k = fan->k[o];
-                            const int f = fan->flags[o];
-                            const float gx0 = dx0 * k[0], gy0 = dy0 * k[1];
-                            const float gx1 = dx1 * k[0], gy1 = dy1 * k[1];
-                            float v0, v1;
-                            switch (fan->op[o]) {
-                            case STRATA_OP_SLOPE:
-                                slope_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
-                                break;
-                            case STRATA_OP_ASPECT:  // the short form: this loop's code stays small
-                                v0 = aspect_of(gx0, gy0, k[2], (f & STRATA_FLAG_TRIG) != 0);
-                                v1 = aspect_of(gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0);
-                                break;
-                            case STRATA_OP_HILLSHADE:
-                                shade_pair(gx0, gy0, gx1, gy1, k[2], k[3], k[4], v0, v1);
-                                break;
-                            default: {  // STRATA_OP_CURVATURE
-                                const int kind = (f >> STRATA_CURV_SHIFT) & 3;
-                                v0 = zt_curvature(A.l, A.c0, A.c1, B.l, B.c0, B.c1, C.l, C.c0, C.c1, k, kind);
-                                v1 = zt_curvature(A.c0, A.c1, A.r, B.c0, B.c1, B.r, C.c0, C.c1, C.r, k, kind);
-                            }
-                            }
-                            store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
-                        }
+            for (int o = 0; o < kMaxOutputs; ++o) {
+                if (o < fan->n) {
+                    const float *k = fan->k[o];
+                    const int f = fan->flags[o];
+                    const float gx0 = dx0 * k[0], gy0 = dy0 * k[1];
+                    const float gx1 = dx1 * k[0], gy1 = dy1 * k[1];
+                    float v0, v1;
+                    switch (fan->op[o]) {
+                    case STRATA_OP_SLOPE:
+                        slope_pair(gx0, gy0, gx1, gy1, k[2], (f & STRATA_FLAG_ATAN) != 0, v0, v1);
+                        break;
+                    case STRATA_OP_ASPECT:  // the short form: this loop's code stays small
+                        v0 = aspect_of(gx0, gy0, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                        v1 = aspect_of(gx1, gy1, k[2], (f & STRATA_FLAG_TRIG) != 0);
+                        break;
+                    case STRATA_OP_HILLSHADE:
+                        shade_pair(gx0, gy0, gx1, gy1, k[2], k[3], k[4], v0, v1);
+                        break;
+                    default: {  // STRATA_OP_CURVATURE
+                        const int kind = (f >> STRATA_CURV_SHIFT) & 3;
+                        v0 = zt_curvature(A.l, A.c0, A.c1, B.l, B.c0, B.c1, C.l, C.c0, C.c1, k, kind);
+                        v1 = zt_curvature(A.c0, A.c1, A.r, B.c0, B.c1, B.r, C.c0, C.c1, C.r, k, kind);
                     }
-                } else {
-                    float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
-                    horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
-                    store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
-                    if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
+                    }
+                    store(dset[o], yy, (ri && ci0) ? v0 : nan, (ri && ci1) ? v1 : nan);
                 }
             }
-        }
+        } else {
+            float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
+            horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
+            store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
+            if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
+        }    };
+    // Whole groups of four rows with no condition inside, so the ring keeps
+    // fixed registers and every step issues the same memory operations; then
+    // the strip's last rows.
+    int32_t ub = 0;
+    for (; ub + 4 <= nrows; ub += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) step(u, ub + u);
     }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+        if (ub + u < nrows) step(u, ub + u);
 }
 
 // UNALIGNED: the operands do not allow pair accesses at even offsets, so
@@ -662,11 +678,26 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     }
 }
 
-// horn_sweep with a chain's point stages applied to every stored value
-// (single-output ops).
+// Copies a chain's point stages into device memory.
+__global__ __launch_bounds__(256) void stage_post(PostArgs pa, PostArgs *out)
+{
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&pa);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(out);
+    for (uint32_t i = threadIdx.x; i < sizeof(PostArgs) / 4; i += 256) dst[i] = src[i];
+}
+
+hipError_t launch_stage_post(const PostArgs &pa, PostArgs *dev, hipStream_t st)
+{
+    static_assert(sizeof(PostArgs) % 4 == 0, "PostArgs is copied in 32-bit words");
+    hipLaunchKernelGGL(stage_post, dim3(1), dim3(256), 0, st, pa, dev);
+    return hipGetLastError();
+}
+
+// horn_sweep with a chain's point stages (in device memory) applied to
+// every stored value (single-output ops).
 template <int OP, bool UNALIGNED>
 __global__ __launch_bounds__(kSweepThreads) void horn_sweep_post(HornArgs a, Region rg, SrcView s, DstView d0,
-                                                                 PostArgs pa)
+                                                                 const PostArgs *__restrict__ pa)
 {
     constexpr uint32_t kXcds = 8;
     const uint32_t nb = gridDim.x, b = blockIdx.x;
@@ -676,7 +707,7 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_post(HornArgs a, Reg
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
     const bool inside = xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
-    const ArgPost post{&pa};
+    const ArgPost post{gp(pa)};
     if constexpr (UNALIGNED) {
         if (inside)
             sweep<OP, kSweepAnyParity>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
@@ -2013,12 +2044,13 @@ static hipError_t run_horn(const Params &P, const Region &rg, const SrcView &s, 
         const dim3 grid((unsigned)(gx * cdiv(sub.h, a.rows)));
         if constexpr (OP != STRATA_OP_GRADIENT) {
             if (P.post.n > 0) {
+                if (!P.post_dev) return hipErrorInvalidValue;
                 if (aligned)
                     hipLaunchKernelGGL((horn_sweep_post<OP, false>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0,
-                                       P.post);
+                                       P.post_dev);
                 else
                     hipLaunchKernelGGL((horn_sweep_post<OP, true>), grid, dim3(kSweepThreads), 0, st, a, sub, s, e0,
-                                       P.post);
+                                       P.post_dev);
                 continue;
             }
         }

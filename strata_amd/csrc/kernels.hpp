@@ -86,7 +86,12 @@ struct Params {
     float taps[kMaxTaps];
     FanArgs fan;
     PostArgs post;
+    const PostArgs *post_dev;  // post in device memory (launch_stage_post), read by the fused sweep
 };
+
+// Writes pa into device memory at dev (a kernel copies it from its own
+// arguments, in stream order).
+hipError_t launch_stage_post(const PostArgs &pa, PostArgs *dev, hipStream_t st);
 
 // Launch-shape knobs (results never depend on them).
 struct Tuning {

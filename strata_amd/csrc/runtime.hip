@@ -445,7 +445,20 @@ int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t
             d.stride = bs;
         }
         const Region rg{W, H, out.x0, out.y0, out.x1 - out.x0, out.y1 - out.y0};
-        HIP_TRY(launch_data(G.P, rg, s, &d, 1, st), "launching a chain stage");
+        if (G.P.post.n > 0) {
+            // The fused point stages' tables go to device memory for the sweep.
+            void *pd = nullptr;
+            HIP_TRY(scratch_alloc_async(&pd, sizeof(PostArgs), st), "a chain's tables");
+            Params P2 = G.P;
+            P2.post_dev = (const PostArgs *)pd;
+            hipError_t e = launch_stage_post(G.P.post, (PostArgs *)pd, st);
+            if (e == hipSuccess) e = launch_data(P2, rg, s, &d, 1, st);
+            const hipError_t f = hipFreeAsync(pd, st);
+            HIP_TRY(e, "launching a chain stage");
+            HIP_TRY(f, "freeing a chain's tables");
+        } else {
+            HIP_TRY(launch_data(G.P, rg, s, &d, 1, st), "launching a chain stage");
+        }
         for (const Params &Q : G.inplace) {
             const SrcView v{d.p, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
             HIP_TRY(launch_data(Q, rg, v, &d, 1, st), "launching a chain's point stage");
