@@ -177,7 +177,10 @@ __device__ __forceinline__ float go_max(float a, float b) { return __builtin_ele
 // keeps the first hit by selects: the tables are read with wave-uniform
 // indices only (scalar loads, no per-lane gathers and no divergent loop),
 // and the value is computed from the same knots in the same operations.
-__device__ __forceinline__ float reclass_of(float v, const float *breaks, int nb, const float *values)
+// T is the tables' pointer type (a generic, global or constant pointer to
+// const float).
+template <class T>
+__device__ __forceinline__ float reclass_of(float v, T breaks, int nb, T values)
 {
     float out = values[nb];
     bool found = false;
@@ -188,7 +191,8 @@ __device__ __forceinline__ float reclass_of(float v, const float *breaks, int nb
     }
     return v != v ? v : out;
 }
-__device__ __forceinline__ float lookup_of(float v, const float *xs, const float *ys, int n)
+template <class T>
+__device__ __forceinline__ float lookup_of(float v, T xs, T ys, int n)
 {
     // state 2: no knot above v (k == n); 0: the first knot (k == 0); 1:
     // between knots k-1 and k, which x0, y0, x1, y1 hold.
@@ -211,6 +215,64 @@ __device__ __forceinline__ float lookup_of(float v, const float *xs, const float
     const float mid = v == x0 ? y0 : y0 + t * (y1 - y0);
     const float out = state == 0 ? ys[0] : (state == 2 ? ys[n - 1] : mid);
     return v != v ? v : out;
+}
+
+// The counting forms of the same two ops. For a table that strictly
+// increases and holds no NaN (the reference's constructors panic on any
+// other, transfer/transfer.go:204-221, and the library's checks refuse
+// them), the entries above a value that is not NaN are a suffix of the
+// table, so the scan's k (the first entry above v) is the number of
+// entries at or below v: one compare and one add per entry, with no
+// selects of the entries' values. The entries k picks are then gathered
+// per lane from a copy of the table in LDS. A NaN v counts every entry
+// (every v < x is false), which keeps k inside the table, and gives v.
+// Each form takes N values, which share every read of the table.
+template <int N, class T>
+__device__ __forceinline__ void count_at_or_below(const float (&v)[N], T xs, int n, int (&k)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) k[i] = 0;
+    for (int j = 0; j < n; j++) {
+        const float x = xs[j];
+#pragma unroll
+        for (int i = 0; i < N; ++i) k[i] += v[i] < x ? 0 : 1;
+    }
+}
+// values: the nb + 1 class values in LDS.
+template <int N, class T, class L>
+__device__ __forceinline__ void reclass_count(float (&v)[N], T breaks, int nb, L values)
+{
+    int k[N];
+    count_at_or_below(v, breaks, nb, k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const float out = values[k[i]];
+        v[i] = v[i] != v[i] ? v[i] : out;
+    }
+}
+// knots: (xs[j], ys[j]) as 2-vectors in LDS, one per knot.
+template <int N, class T, class L2>
+__device__ __forceinline__ void lookup_count(float (&v)[N], T xs, T ys, int n, L2 knots)
+{
+    if (n == 1) {  // every value is below, on or above the one knot
+        const float y = ys[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = v[i] != v[i] ? v[i] : y;
+        return;
+    }
+    int k[N];
+    count_at_or_below(v, xs, n, k);
+    const float ylo = ys[0], yhi = ys[n - 1];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int kk = min(max(k[i], 1), n - 1);  // the segment for 0 < k < n; any segment otherwise
+        const auto p0 = knots[kk - 1], p1 = knots[kk];
+        const float x0 = p0.x, y0 = p0.y;
+        const float t = (v[i] - x0) / (p1.x - x0);
+        const float mid = v[i] == x0 ? y0 : y0 + t * (p1.y - y0);
+        const float out = k[i] == 0 ? ylo : (k[i] == n ? yhi : mid);
+        v[i] = v[i] != v[i] ? v[i] : out;
+    }
 }
 
 } // namespace strata

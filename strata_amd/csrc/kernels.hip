@@ -48,6 +48,16 @@ __device__ __forceinline__ const STRATA_GLOBAL T *gp(const T *p)
 {
     return (const STRATA_GLOBAL T *)p;
 }
+// Constant memory: device memory that no access of the launch writes. Loads
+// through such a pointer are scalar and, unlike global loads, are not
+// ordered against the launch's stores, so the compiler keeps what it loaded
+// across them instead of loading it again after each one.
+#define STRATA_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const STRATA_CONST T *cp(const T *p)
+{
+    return (const STRATA_CONST T *)p;
+}
 typedef float pair_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float load_cell(const SrcView &s, int64_t x, int64_t y)
@@ -360,30 +370,72 @@ __device__ __forceinline__ void horn_pair(const HornArgs &a, const Row &A, const
     }
 }
 
-// The identity on a sweep's output values.
+// The identity on a sweep's output values (a lane's two).
 struct NoPost {
-    __device__ float operator()(float v) const { return v; }
+    __device__ void operator()(float &, float &) const {}
 };
 
-// A chain's point stages on a sweep's output values, in order (Lookup and
-// Reclass as internal/curve/scalar.go:39-114 through arith.hpp), so a
-// stencil followed by point stages writes the chain's value straight from
-// registers. The tables are kernel arguments, read as uniform values.
-struct ArgPost {
-    const STRATA_GLOBAL PostArgs *a;
-    __device__ float operator()(float v) const
+// LDS memory, for pointers that reach a function through a structure.
+#define STRATA_LDS __attribute__((address_space(3)))
+
+// A point op's table in LDS for the counting forms of Lookup and Reclass
+// (arith.hpp): a Lookup's knots as (x, y) pairs, a Reclass's nb + 1 class
+// values. Each copy takes an even number of floats, so copies placed one
+// after another from an 8-byte-aligned start keep their pairs aligned.
+__device__ __forceinline__ int lds_table_floats(int op, int cnt)
+{
+    return op == STRATA_OP_LOOKUP ? cnt : (((cnt - 1) / 2 + 2) & ~1);
+}
+// Copies op's table (cnt floats at t in the C ABI's layout: xs then ys, or
+// breaks then values) into lt. Every thread of the block calls it, before
+// a barrier; tid is the thread's index among nthreads.
+template <class T>
+__device__ __forceinline__ void lds_table(STRATA_LDS float *lt, int op, T t, int cnt, int tid, int nthreads)
+{
+    if (op == STRATA_OP_LOOKUP) {
+        const int n = cnt / 2;
+        for (int m = tid; m < cnt; m += nthreads) lt[m] = t[(m & 1) ? n + (m >> 1) : (m >> 1)];
+    } else {
+        const int nb = (cnt - 1) / 2;
+        for (int m = tid; m <= nb; m += nthreads) lt[m] = t[nb + m];
+    }
+}
+// op's values of v, in place (internal/curve/scalar.go:39-114): the
+// table's x side is read at uniform indices from t, the entries picked are
+// gathered from its LDS copy lt.
+template <int N, class T>
+__device__ __forceinline__ void point_values(float (&v)[N], int op, T t, int cnt, const STRATA_LDS float *lt)
+{
+    if (op == STRATA_OP_LOOKUP) {
+        const int n = cnt / 2;
+        lookup_count(v, t, t + n, n, reinterpret_cast<const STRATA_LDS pair_t *>(lt));
+    } else {
+        const int nb = (cnt - 1) / 2;
+        reclass_count(v, t, nb, lt);
+    }
+}
+
+// A chain's point stages on a sweep's output values, in order, so a stencil
+// followed by point stages writes the chain's value straight from
+// registers. The tables are in device memory that the launch only reads
+// (x sides read as uniform values) and in the block's LDS, one copy after
+// another in stage order (lds_table).
+constexpr int kPostLdsFloats = kPostTaps + 2 * kMaxPost;
+struct LdsPost {
+    const STRATA_CONST PostArgs *a;
+    const STRATA_LDS float *lt;
+    // A lane's two values.
+    __device__ void operator()(float &v0, float &v1) const
     {
+        float v[2] = {v0, v1};
+        int lo = 0;
         for (int i = 0; i < a->n; ++i) {
-            const float *t = (const float *)(a->taps + a->off[i]);
-            if (a->op[i] == STRATA_OP_LOOKUP) {
-                const int n = a->cnt[i] / 2;
-                v = lookup_of(v, t, t + n, n);
-            } else {
-                const int nb = (a->cnt[i] - 1) / 2;
-                v = reclass_of(v, t, nb, t + nb);
-            }
+            const int op = a->op[i], c = a->cnt[i];
+            point_values(v, op, a->taps + a->off[i], c, lt + lo);
+            lo += lds_table_floats(op, c);
         }
-        return v;
+        v0 = v[0];
+        v1 = v[1];
     }
 };
 
@@ -628,7 +680,9 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
         } else {
             float o00 = 0.0f, o01 = 0.0f, o10 = 0.0f, o11 = 0.0f;
             horn_pair<OP>(a, A, B, C, o00, o01, o10, o11);
-            store(d0, yy, post((ri && ci0) ? o00 : nan), post((ri && ci1) ? o01 : nan));
+            float p0 = (ri && ci0) ? o00 : nan, p1 = (ri && ci1) ? o01 : nan;
+            post(p0, p1);
+            store(d0, yy, p0, p1);
             if constexpr (kTwo) store(d1, yy, (ri && ci0) ? o10 : nan, (ri && ci1) ? o11 : nan);
         }    };
     // Whole groups of four rows with no condition inside, so the ring keeps
@@ -707,7 +761,16 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_post(HornArgs a, Reg
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
     const bool inside = xb >= s.lx0 && xb + kSweepCols <= s.lx1 && xb >= rg.ox && xb + kSweepCols <= rg.ox + rg.w;
-    const ArgPost post{gp(pa)};
+    __shared__ __align__(16) float lt[kPostLdsFloats];
+    const STRATA_CONST PostArgs *pc = cp(pa);
+    int lo = 0;
+    for (int i = 0; i < pc->n; ++i) {
+        const int op = pc->op[i], c = pc->cnt[i];
+        lds_table((STRATA_LDS float *)lt + lo, op, pc->taps + pc->off[i], c, (int)threadIdx.x, kSweepThreads);
+        lo += lds_table_floats(op, c);
+    }
+    __syncthreads();
+    const LdsPost post{pc, (const STRATA_LDS float *)lt};
     if constexpr (UNALIGNED) {
         if (inside)
             sweep<OP, kSweepAnyParity>(a, rg, s, d0, d0, xb, y0, y1, threadIdx.x, post);
@@ -1414,24 +1477,33 @@ __global__ __launch_bounds__(256) void correlate_error_f64(const Params P, Regio
 
 constexpr int kPointRows = 16;
 
+// The table is read at uniform indices from the kernel arguments and
+// gathered from the block's LDS copy.
 template <int OP>
 __global__ __launch_bounds__(256) void point_rows(const Params P, Region rg, SrcView s, DstView d0)
 {
+    __shared__ __align__(16) float lt[kMaxTaps + 2];
+    lds_table((STRATA_LDS float *)lt, OP, P.taps, P.ntaps, (int)threadIdx.x, 256);
+    __syncthreads();
     const int64_t x = rg.ox + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x >= rg.ox + rg.w) return;
     const int64_t y0 = rg.oy + (int64_t)blockIdx.y * kPointRows;
     const int64_t y1 = min(y0 + (int64_t)kPointRows, rg.oy + rg.h);
+    const STRATA_LDS float *tl = (const STRATA_LDS float *)lt;
+    if (y1 - y0 == kPointRows) {
+        // The column's cells all loaded, then one pass over the table.
+        float v[kPointRows];
+#pragma unroll
+        for (int i = 0; i < kPointRows; ++i) v[i] = gp(s.p)[(y0 + i - s.sy) * s.stride + (x - s.sx)];
+        point_values(v, OP, P.taps, P.ntaps, tl);
+#pragma unroll
+        for (int i = 0; i < kPointRows; ++i) gp(d0.p)[(y0 + i - rg.oy) * d0.stride + (x - rg.ox)] = v[i];
+        return;
+    }
     for (int64_t y = y0; y < y1; ++y) {
-        const float v = gp(s.p)[(y - s.sy) * s.stride + (x - s.sx)];
-        float o;
-        if constexpr (OP == STRATA_OP_LOOKUP) {
-            const int n = P.ntaps / 2;
-            o = lookup_of(v, P.taps, P.taps + n, n);
-        } else {
-            const int nb = (P.ntaps - 1) / 2;
-            o = reclass_of(v, P.taps, nb, P.taps + nb);
-        }
-        gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = o;
+        float v[1] = {gp(s.p)[(y - s.sy) * s.stride + (x - s.sx)]};
+        point_values(v, OP, P.taps, P.ntaps, tl);
+        gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = v[0];
     }
 }
 
@@ -1770,16 +1842,21 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
 // task's fields are scalar.
 
 // The point ops of a batch op, applied to a stencil op's output values. The
-// count is read once: the sweep's stores could alias the op table as far as
-// the compiler knows, which would reload it after every store.
+// op table is read-only for the launch and read as constant memory, so the
+// sweep's stores do not make the compiler load it again after each one.
 struct ChainPost {
-    const BatchOp *op;
+    const STRATA_CONST BatchOp *op;
     int npost;
-    __device__ explicit ChainPost(const BatchOp *o) : op(o), npost(o->npost) {}
-    __device__ float operator()(float v) const
+    __device__ explicit ChainPost(const BatchOp *o) : op(cp(o)), npost(cp(o)->npost) {}
+    __device__ void operator()(float &v0, float &v1) const
+    {
+        v0 = one(v0);
+        v1 = one(v1);
+    }
+    __device__ float one(float v) const
     {
         for (int i = 0; i < npost; ++i) {
-            const float *t = op->post_taps[i];
+            const STRATA_CONST float *t = op->post_taps[i];
             if (op->post_op[i] == STRATA_OP_LOOKUP) {
                 const int n = op->post_n[i] / 2;
                 v = lookup_of(v, t, t + n, n);

@@ -113,6 +113,26 @@ int check_kernel(const strata_kernel *k, int ndst, int nsrc)
     if (k->ntaps < 0 || k->ntaps > kMaxTaps)
         return fail(STRATA_EINVAL, "engine: %d taps exceed the device limit of %d", k->ntaps, kMaxTaps);
     if (k->ntaps > 0 && !k->taps) return fail(STRATA_EINVAL, "engine: kernel has %d taps and a nil pointer", k->ntaps);
+    if (k->op == STRATA_OP_LOOKUP || k->op == STRATA_OP_RECLASS) {
+        // transfer/transfer.go:176-221: the x side holds no NaN and strictly
+        // increases (the kernels count the entries at or below a value,
+        // which is the scan's index only for such a table), and a Lookup's
+        // xs are finite.
+        const bool lookup = k->op == STRATA_OP_LOOKUP;
+        const char *op = lookup ? "transfer.Lookup" : "transfer.Reclass", *name = lookup ? "xs" : "breaks";
+        const int n = lookup ? k->ntaps / 2 : (k->ntaps - 1) / 2;
+        const float *x = k->taps;
+        for (int i = 0; i < n; ++i)
+            if (x[i] != x[i]) return fail(STRATA_EINVAL, "%s: %s must not hold NaN, but %s[%d] is NaN", op, name, name, i);
+        for (int i = 1; i < n; ++i)
+            if (!(x[i - 1] < x[i]))
+                return fail(STRATA_EINVAL, "%s: %s must be strictly increasing, but %s[%d] = %g is not above %s[%d] = %g",
+                            op, name, name, i, (double)x[i], name, i - 1, (double)x[i - 1]);
+        if (lookup)
+            for (int i = 0; i < n; ++i)
+                if (x[i] - x[i] != 0.0f)
+                    return fail(STRATA_EINVAL, "%s: xs must be finite, but xs[%d] = %g", op, i, (double)x[i]);
+    }
     if (nsrc != 1 || ndst != n_outputs(k))
         return fail(STRATA_EINVAL, "engine: kernel takes %d inputs and %d outputs, got %d and %d", 1, n_outputs(k),
                     nsrc, ndst);
@@ -314,6 +334,36 @@ Job make_chain_job(const strata_kernel *stages, int n)
     return J;
 }
 
+// The tables of a chain's fused point stages, in device memory for one
+// call: written once on the call's stream before its launches (every launch
+// of the call is on that stream) and freed after them in stream order.
+struct ChainTables {
+    void *dev = nullptr;
+    hipStream_t st = nullptr;
+    ChainTables() = default;
+    ChainTables(const ChainTables &) = delete;
+    ChainTables &operator=(const ChainTables &) = delete;
+    ~ChainTables()
+    {
+        if (dev) (void)hipFreeAsync(dev, st);
+    }
+    int stage(Job &J, hipStream_t s)
+    {
+        size_t n = 0;
+        for (const ChainGroup &G : J.chain) n += G.P.post.n > 0;
+        if (n == 0) return STRATA_OK;
+        st = s;
+        HIP_TRY(scratch_alloc_async(&dev, n * sizeof(PostArgs), st), "a chain's tables");
+        PostArgs *p = static_cast<PostArgs *>(dev);
+        for (ChainGroup &G : J.chain) {
+            if (G.P.post.n == 0) continue;
+            HIP_TRY(launch_stage_post(G.P.post, p, st), "staging a chain's tables");
+            G.P.post_dev = p++;
+        }
+        return STRATA_OK;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Rasters
 // ---------------------------------------------------------------------------
@@ -445,20 +495,9 @@ int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t
             d.stride = bs;
         }
         const Region rg{W, H, out.x0, out.y0, out.x1 - out.x0, out.y1 - out.y0};
-        if (G.P.post.n > 0) {
-            // The fused point stages' tables go to device memory for the sweep.
-            void *pd = nullptr;
-            HIP_TRY(scratch_alloc_async(&pd, sizeof(PostArgs), st), "a chain's tables");
-            Params P2 = G.P;
-            P2.post_dev = (const PostArgs *)pd;
-            hipError_t e = launch_stage_post(G.P.post, (PostArgs *)pd, st);
-            if (e == hipSuccess) e = launch_data(P2, rg, s, &d, 1, st);
-            const hipError_t f = hipFreeAsync(pd, st);
-            HIP_TRY(e, "launching a chain stage");
-            HIP_TRY(f, "freeing a chain's tables");
-        } else {
-            HIP_TRY(launch_data(G.P, rg, s, &d, 1, st), "launching a chain stage");
-        }
+        // A group with fused point stages reads their tables at G.P.post_dev,
+        // staged once for the call (ChainTables).
+        HIP_TRY(launch_data(G.P, rg, s, &d, 1, st), "launching a chain stage");
         for (const Params &Q : G.inplace) {
             const SrcView v{d.p, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
             HIP_TRY(launch_data(Q, rg, v, &d, 1, st), "launching a chain's point stage");
@@ -2010,7 +2049,11 @@ int strata_process_chain(const strata_kernel *stages, int nstages, const strata_
 {
     int rc = check_chain(stages, nstages);
     if (rc) return rc;
-    return process(make_chain_job(stages, nstages), dst, 1, src, 1, opts, stats);
+    if ((rc = ensure_device())) return rc;
+    Job J = make_chain_job(stages, nstages);
+    ChainTables T;
+    if ((rc = T.stage(J, opts ? (hipStream_t)opts->stream : nullptr))) return rc;
+    return process(J, dst, 1, src, 1, opts, stats);
 }
 
 int strata_process_chain_chunked(const strata_kernel *stages, int nstages, const strata_sink *dst,
@@ -2018,7 +2061,11 @@ int strata_process_chain_chunked(const strata_kernel *stages, int nstages, const
 {
     int rc = check_chain(stages, nstages);
     if (rc) return rc;
-    return process_chunked(make_chain_job(stages, nstages), dst, 1, src, 1, opts, stats);
+    if ((rc = ensure_device())) return rc;
+    Job J = make_chain_job(stages, nstages);
+    ChainTables T;
+    if ((rc = T.stage(J, opts ? (hipStream_t)opts->stream : nullptr))) return rc;
+    return process_chunked(J, dst, 1, src, 1, opts, stats);
 }
 
 } // extern "C"

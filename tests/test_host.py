@@ -334,3 +334,47 @@ def test_chain_checks():
     too_far = engine.Pipeline([focal.newBox(focal.BoxOptions(5), "mean"), focal.newBox(focal.BoxOptions(4), "max")])
     with pytest.raises(EnginePanic, match="reaches 9"):
         PlanStats(too_far, 100, 100)
+
+
+@pytest.mark.parametrize("op,xs,rest,msg", [
+    (_lib.OP_LOOKUP, [0, float("nan"), 2], [1, 2, 3], "transfer.Lookup: xs must not hold NaN, but xs[1] is NaN"),
+    (_lib.OP_LOOKUP, [0, 2, 2], [1, 2, 3], "transfer.Lookup: xs must be strictly increasing, but xs[2] = 2 is not "
+                                           "above xs[1] = 2"),
+    (_lib.OP_LOOKUP, [1, 0], [1, 2], "transfer.Lookup: xs must be strictly increasing, but xs[1] = 0 is not above"),
+    (_lib.OP_LOOKUP, [-0.0, 0.0], [1, 2], "transfer.Lookup: xs must be strictly increasing"),
+    (_lib.OP_LOOKUP, [0, float("inf")], [1, 2], "transfer.Lookup: xs must be finite, but xs[1] = inf"),
+    (_lib.OP_LOOKUP, [float("-inf")], [1], "transfer.Lookup: xs must be finite, but xs[0] = -inf"),
+    (_lib.OP_RECLASS, [float("nan")], [1, 2], "transfer.Reclass: breaks must not hold NaN, but breaks[0] is NaN"),
+    (_lib.OP_RECLASS, [3, 1], [1, 2, 3], "transfer.Reclass: breaks must be strictly increasing, but breaks[1] = 1"),
+])
+def test_c_abi_refuses_tables_the_reference_refuses(op, xs, rest, msg):
+    """The C ABI checks Lookup and Reclass tables as the reference's
+    constructors do (transfer/transfer.go:176-221): the device kernels count
+    the entries at or below a value, which equals the reference's scan only
+    for a strictly increasing table free of NaN."""
+    taps = np.asarray(list(xs) + list(rest), np.float32)
+    k = _lib.Kernel()
+    k.op, k.radius, k.ntaps, k.taps = op, 0, len(taps), taps.ctypes.data
+    st = _lib.Stats()
+    rc = _lib.lib().strata_plan_stats(ctypes_ref(k), 1, 1, 10, 10, None, 0, ctypes_ref(st))
+    assert rc == _lib.EINVAL
+    assert _lib.last_error().startswith(msg)
+
+
+@pytest.mark.parametrize("op,taps", [
+    (_lib.OP_LOOKUP, [0, 1]),                                   # one knot
+    (_lib.OP_LOOKUP, [-1e30, 1e30, float("nan"), float("inf")]),  # ys are not checked
+    (_lib.OP_RECLASS, [5]),                                     # no breaks
+    (_lib.OP_RECLASS, [float("-inf"), float("inf"), 1, 2, 3]),  # infinite breaks are ordered values
+])
+def test_c_abi_accepts_tables_the_reference_accepts(op, taps):
+    taps = np.asarray(taps, np.float32)
+    k = _lib.Kernel()
+    k.op, k.radius, k.ntaps, k.taps = op, 0, len(taps), taps.ctypes.data
+    st = _lib.Stats()
+    assert _lib.lib().strata_plan_stats(ctypes_ref(k), 1, 1, 10, 10, None, 0, ctypes_ref(st)) == 0
+
+
+def ctypes_ref(x):
+    import ctypes
+    return ctypes.byref(x)
