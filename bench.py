@@ -83,6 +83,10 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the Slope kernel's HBM traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-op", default="slope",
+                    choices=("slope", "mean3", "correlate1", "gauss2", "correlate8-mfma", "correlate4-mfma"),
+                    help="with --pmc-child: the op run twice before the calibration copy (focal ops over the "
+                         "--focal-size raster of the focal workload)")
     return ap.parse_args()
 
 
@@ -575,12 +579,14 @@ def folded_bench(args, dem, N, dev, log):
 
 
 def pmc_child(args) -> int:
-    """The workload of one counter pass: the Slope op twice on the same DEM,
-    then one device copy of the DEM (the calibration kernel)."""
+    """The workload of one counter pass: the op twice on the same input,
+    then one device copy of the input (the calibration kernel). Slope runs
+    over the DEM as ProcessChunked with --tile-height; the focal ops over
+    the focal workload's raster as ProcessN."""
     import torch
 
-    from strata_amd import _lib, synth, terrain
-    from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked
+    from strata_amd import _lib, focal, synth, terrain
+    from strata_amd.engine import MemorySink, MemorySource, Options, ProcessChunked, ProcessN
     from strata_amd.raster import Float32Raster
 
     dev = torch.device("cuda:0")
@@ -591,6 +597,29 @@ def pmc_child(args) -> int:
         L.strata_get_tuning(t)
         t.sweep_rows = args.sweep_rows
         assert L.strata_set_tuning(t) == 0, _lib.last_error()
+    if args.pmc_op != "slope":
+        import numpy as np
+        S = args.focal_size
+        data = synth.uniform_raster(S * S, 0.0, 1000.0, seed=3, device=dev)
+        valid = synth.random_nodata(S * S, 0.02, seed=4, device=dev)
+        src = Float32Raster(data, S, S, S, valid, 0)
+        dst = Float32Raster(torch.empty(S * S, dtype=torch.float32, device=dev), S, S, S, torch.zeros_like(valid), 0)
+        r = {"mean3": 3, "correlate1": 1, "gauss2": 2, "correlate8-mfma": 8, "correlate4-mfma": 4}[args.pmc_op]
+        n = 2 * r + 1
+        if args.pmc_op == "mean3":
+            k = focal.newBox(focal.BoxOptions(3), "mean")
+        elif args.pmc_op == "gauss2":
+            g = focal.Gaussian(r, r / 2)
+            k = focal.newSeparableKernel(focal.SeparableOptions(r, g, g))
+        else:
+            w = np.random.default_rng(5).uniform(-1, 1, n * n).astype(np.float32)
+            k = focal.newWeightsKernel(focal.WeightsOptions(r, w), False, mfma=args.pmc_op.endswith("mfma"))
+        for _ in range(2):
+            assert ProcessN(None, [dst], [src], k, Options()) is None
+        tmp = torch.empty_like(data)
+        tmp.copy_(data)
+        torch.cuda.synchronize()
+        return 0
     N = args.size
     data = synth.fbm_dem(N, N, hurst=0.8, seed=1, lo=0.0, hi=3000.0, device=dev)
     valid, _ = synth.blob_nodata(N, N, frac=0.005, rmin=5, rmax=50, seed=2, device=dev)

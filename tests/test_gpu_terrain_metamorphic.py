@@ -188,9 +188,12 @@ def test_negation_is_negative_zfactor():
 
 def test_scaling_and_negation_at_20000():
     """Power-of-two scaling against ZFactor and negation against a negative
-    ZFactor, for Slope and Hillshade over the 20000² fBm DEM with 0.5%
-    NoData blobs (the headline workload), compared bit for bit on the
-    device."""
+    ZFactor, for Slope, Aspect, Hillshade and the three curvatures over the
+    20000² fBm DEM with 0.5% NoData blobs (the terrain-suite workload),
+    compared bit for bit on the device. Every op sees the elevations only
+    through differences scaled by constants linear in ZFactor, so scaling
+    the DEM by 2^k and ZFactor by 2^-k (or negating both) leaves each
+    derivative, and so each output, unchanged."""
     from strata_amd import synth
 
     N = 20000
@@ -205,8 +208,13 @@ def test_scaling_and_negation_at_20000():
         return out
 
     bits = ((valid.view(-1, 1) >> torch.arange(64, device=dev)) & 1).reshape(-1)[:N * N].bool()
+    curv = lambda t: lambda z: T.curvatureOp(T.CurvatureOptions(CellSize=30, ZFactor=z, Type=t))
     for name, mk in (("slope", lambda z: T.slopeOp(T.SlopeOptions(CellSize=30, ZFactor=z))),
-                     ("hillshade", lambda z: T.hillshadeOp(T.HillshadeOptions(CellSize=30, ZFactor=z)))):
+                     ("aspect", lambda z: T.aspectOp(T.AspectOptions(CellSize=30, ZFactor=z))),
+                     ("hillshade", lambda z: T.hillshadeOp(T.HillshadeOptions(CellSize=30, ZFactor=z))),
+                     ("curvature-profile", curv(T.CurvatureProfile)),
+                     ("curvature-plan", curv(T.CurvaturePlan)),
+                     ("curvature-mean", curv(T.CurvatureMean))):
         base = go(mk(1.0), data)
         ok = base.Valid
         want = base.Data[bits]
