@@ -136,15 +136,22 @@ def main() -> int:
         assert err is None
 
     def time_op(k, sinks, steps, warmup):
+        """The steps timed as a caller runs them; then the kernels' own
+        durations over as many steps again, with the library's timing events
+        around each launch (which add a marker between launches, so they
+        stay out of the timed steps)."""
         for _ in range(warmup):
             run_op(k, sinks)
         torch.cuda.synchronize()
-        timing = _lib.KernelTiming().begin()
         t = time.perf_counter()
         for _ in range(steps):
             run_op(k, sinks)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
+        timing = _lib.KernelTiming().begin()
+        for _ in range(steps):
+            run_op(k, sinks)
+        torch.cuda.synchronize()
         timing.end()
         return el, timing
 
@@ -635,59 +642,73 @@ def pmc_child(args) -> int:
     return 0
 
 
+SIZED_READS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum")
+
+
 def pmc_traffic(args, N, log):
     """HBM bytes per launch of the Slope kernel from two rocprofv3
     counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
     scaled by what the same counter reads for a plain device copy of the
-    DEM (4·N² bytes read and written) in the same run."""
+    DEM (4·N² bytes read and written) in the same run. A third pass counts
+    the read requests by size (32, 64 and 128 bytes): FETCH_SIZE counts a
+    128-byte request as 64 bytes on this part, so the copy's factor holds
+    for a kernel only if its requests have the copy's sizes; the sized
+    count needs no factor, and the copy checks it."""
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(exe):
         return {"error": "rocprofv3 not found"}
     copy_kib = 4.0 * N * N / 1024.0
     got = {}
+    import csv
     with tempfile.TemporaryDirectory(prefix="strata_pmc_") as d:
-        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            out = os.path.join(d, ctr)
-            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "run", "--",
+        for name, ctrs in (("FETCH_SIZE", ["FETCH_SIZE"]), ("WRITE_SIZE", ["WRITE_SIZE"]), ("SIZED", list(SIZED_READS))):
+            out = os.path.join(d, name)
+            cmd = [exe, "--pmc", *ctrs, "--output-format", "csv", "-d", out, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--pmc-child", "--size", str(N),
                    "--tile-height", str(args.tile_height)]
             if args.sweep_rows:
                 cmd += ["--sweep-rows", str(args.sweep_rows)]
-            log(f"bench: counter pass {ctr}")
+            log(f"bench: counter pass {' '.join(ctrs)}")
             try:
                 p = subprocess.run(cmd, capture_output=True, text=True, timeout=150)
             except subprocess.TimeoutExpired:
-                return {"error": f"rocprofv3 --pmc {ctr} timed out"}
+                return {"error": f"rocprofv3 --pmc {name} timed out"}
             files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
             if p.returncode != 0 or not files:
-                return {"error": f"rocprofv3 --pmc {ctr} failed ({p.returncode})"}
-            import csv
-            slope_vals, copy_vals = [], []
+                return {"error": f"rocprofv3 --pmc {name} failed ({p.returncode})"}
+            # value per dispatch: the counters' weighted sum (bytes for SIZED, KiB otherwise)
+            weight = {"TCC_EA0_RDREQ_32B_sum": 32.0, "TCC_EA0_RDREQ_64B_sum": 64.0, "TCC_EA0_RDREQ_128B_sum": 128.0,
+                      "FETCH_SIZE": 1.0, "WRITE_SIZE": 1.0}
+            slope, copy = {}, {}
             with open(files[0]) as f:
                 for row in csv.DictReader(f):
-                    name = row.get("Kernel_Name", "")
-                    v = float(row["Counter_Value"])
+                    kname = row.get("Kernel_Name", "")
+                    v = float(row["Counter_Value"]) * weight[row["Counter_Name"]]
                     # The aligned Slope sweep, with or without the validity waves.
-                    if any(k in name for k in ("horn_sweep<2, false>", "horn_sweep<2>", "horn_sweep_valid<2, false>",
-                                               "horn_sweep_valid<2>")):
-                        slope_vals.append(v)
-                    elif "copyBuffer" in name:
-                        copy_vals.append(v)
-            if not slope_vals or not copy_vals:
-                return {"error": f"rocprofv3 --pmc {ctr}: kernels not found"}
-            got[ctr] = (sum(slope_vals) / len(slope_vals), max(copy_vals))
+                    if any(k in kname for k in ("horn_sweep<2, false>", "horn_sweep<2>", "horn_sweep_valid<2, false>",
+                                                "horn_sweep_valid<2>")):
+                        slope[row["Dispatch_Id"]] = slope.get(row["Dispatch_Id"], 0.0) + v
+                    elif "copyBuffer" in kname:
+                        copy[row["Dispatch_Id"]] = copy.get(row["Dispatch_Id"], 0.0) + v
+            if not slope or not copy:
+                return {"error": f"rocprofv3 --pmc {name}: kernels not found"}
+            got[name] = (sum(slope.values()) / len(slope), max(copy.values()))
     ff = copy_kib / got["FETCH_SIZE"][1]
     wf = copy_kib / got["WRITE_SIZE"][1]
     fetch = got["FETCH_SIZE"][0] * ff * 1024
     write = got["WRITE_SIZE"][0] * wf * 1024
+    sized_read, sized_copy = got["SIZED"]
     res = {"bytes_per_launch": round(fetch + write), "read_bytes": round(fetch), "write_bytes": round(write),
            "fetch_kib_raw": round(got["FETCH_SIZE"][0], 1), "write_kib_raw": round(got["WRITE_SIZE"][0], 1),
            "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+           "read_bytes_by_request_size": round(sized_read),
+           "copy_read_bytes_by_request_size_over_actual": round(sized_copy / (copy_kib * 1024), 4),
            "calibration": f"device copy of the {N}x{N} DEM ({copy_kib:.0f} KiB read and written), same run",
            "algorithmic_bytes": 8.25 * N * N, "algorithmic_note": "4 B in and out per cell and 1/8 B of mask "
                                                                    "in and out (the sweep writes the validity)"}
     log(f"bench: traffic {res['bytes_per_launch'] / 1e9:.3f} GB/launch (read {fetch / 1e9:.3f}, write "
-        f"{write / 1e9:.3f}; factors {ff:.3f}/{wf:.3f})")
+        f"{write / 1e9:.3f}; factors {ff:.3f}/{wf:.3f}); reads by request size {sized_read / 1e9:.3f} GB "
+        f"(copy check {res['copy_read_bytes_by_request_size_over_actual']})")
     return res
 
 
@@ -826,12 +847,17 @@ def focal_bench(args, dev, log):
     for name, k, flops in ops:
         assert ProcessN(None, [dst], [src], k, Options()) is None  # warm-up
         torch.cuda.synchronize()
-        timing = _lib.KernelTiming().begin()
         t = time.perf_counter()
         for _ in range(args.focal_steps):
             assert ProcessN(None, [dst], [src], k, Options(), async_=True) is None
         torch.cuda.synchronize()
         el = (time.perf_counter() - t) / args.focal_steps
+        # The kernels' durations, in separate steps (the timing events stay
+        # out of the timed ones).
+        timing = _lib.KernelTiming().begin()
+        for _ in range(args.focal_steps):
+            assert ProcessN(None, [dst], [src], k, Options(), async_=True) is None
+        torch.cuda.synchronize()
         timing.end()
         kms = timing.data_ms / max(1, timing.data_launches)
         res["ops"][name] = {"gcells_s": round(cells / el / 1e9, 2), "ms": round(el * 1e3, 4),

@@ -21,6 +21,8 @@
 //                region are updated with atomics on the region's own bits.
 //
 // The arithmetic is arith.hpp's, in the reference's order.
+#include <algorithm>
+
 #include "arith.hpp"
 #include "kernels.hpp"
 
@@ -59,6 +61,18 @@ __device__ __forceinline__ const STRATA_CONST T *cp(const T *p)
     return (const STRATA_CONST T *)p;
 }
 typedef float pair_t __attribute__((ext_vector_type(2)));
+
+// The tile workgroup b of nb takes. Workgroups are dealt out to the eight
+// XCDs in turn (b runs on XCD b % 8), each with its own L2; numbered this
+// way, each XCD takes one contiguous run of tiles in row-major order, so
+// tiles that share halo lines (a row's neighbours, and the next tile row
+// soon after) read them through the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
+{
+    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
+    return xcd * per + min(xcd, extra) + q;
+}
 
 __device__ __forceinline__ float load_cell(const SrcView &s, int64_t x, int64_t y)
 {
@@ -711,10 +725,7 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep(HornArgs a, Region r
     // order: side-by-side tiles share the cache lines of their halo columns,
     // and tiles one above the other their halo rows, which then stay in the
     // same L2 instead of being fetched from HBM once per XCD.
-    constexpr uint32_t kXcds = 8;
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
-    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
@@ -753,10 +764,7 @@ template <int OP, bool UNALIGNED>
 __global__ __launch_bounds__(kSweepThreads) void horn_sweep_post(HornArgs a, Region rg, SrcView s, DstView d0,
                                                                  const PostArgs *__restrict__ pa)
 {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
-    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
@@ -792,10 +800,7 @@ struct DstSet {
 // terrain.Surface at radius 1: horn_sweep's layout, every wanted product.
 __global__ __launch_bounds__(kSweepThreads) void horn_surface(HornArgs a, Region rg, SrcView s, DstSet d)
 {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
-    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
@@ -811,10 +816,7 @@ __global__ __launch_bounds__(kSweepThreads) void horn_surface(HornArgs a, Region
 // f.op[o] over the shared rows.
 __global__ __launch_bounds__(kSweepThreads) void horn_fan(HornArgs a, FanArgs f, Region rg, SrcView s, DstSet d)
 {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
-    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
@@ -1199,8 +1201,9 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
     a.flags = P.flags;
 #pragma unroll
     for (int i = 0; i < 12; ++i) a.k[i] = P.k[i];
-    const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kTileX;
-    const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kTileY;
+    const uint32_t gx = (uint32_t)((rg.w + kTileX - 1) / kTileX), t = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t tx0 = rg.ox + (int64_t)(t % gx) * kTileX;
+    const int64_t ty0 = rg.oy + (int64_t)(t / gx) * kTileY;
     focal_block<KIND, R>(a, ArgTaps{P}, rg, s, d.v[0], d.v[1], tx0, ty0, lds, d.v);
 }
 
@@ -1308,8 +1311,9 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     __shared__ float wts[N * N];
     __shared__ int nonfinite;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t tx0 = rg.ox + (int64_t)blockIdx.x * kMfmaTile;
-    const int64_t ty0 = rg.oy + (int64_t)blockIdx.y * kMfmaTile;
+    const uint32_t gx = (uint32_t)((rg.w + kMfmaTile - 1) / kMfmaTile), t = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t tx0 = rg.ox + (int64_t)(t % gx) * kMfmaTile;
+    const int64_t ty0 = rg.oy + (int64_t)(t / gx) * kMfmaTile;
     if (tid == 0) nonfinite = 0;
     for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
     __syncthreads();
@@ -1707,10 +1711,7 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_valid(HornArgs a, Re
         validity_strip<1>(rg, vsrc, has_src, vdst, wx, y0, y1);
         return;
     }
-    constexpr uint32_t kXcds = 8;
-    const uint32_t nb = data_blocks;
-    const uint32_t xcd = b % kXcds, q = b / kXcds, per = nb / kXcds, extra = nb % kXcds;
-    const uint32_t t = xcd * per + min(xcd, extra) + q;
+    const uint32_t t = xcd_tile(b, data_blocks);
     const int64_t xb = rg.ox + (int64_t)(t % a.blocks_x) * kSweepCols;
     const int64_t y0 = rg.oy + (int64_t)(t / a.blocks_x) * a.rows;
     const int64_t y1 = min(y0 + (int64_t)a.rows, rg.oy + rg.h);
@@ -2076,6 +2077,8 @@ __global__ __launch_bounds__(256) void batch_focal_kernel(const BatchHeader *__r
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 constexpr int64_t kMaxGridY = 65535;
+// Workgroups of a 1-D tile launch: gridDim.x times 256 threads stays below 2^32.
+constexpr int64_t kMaxTileBlocks = int64_t(1) << 23;
 
 // Launch-shape tuning (never changes a result).
 static Tuning g_tuning{16, 4096, 0, {0, 0, 0, 0, 0}};
@@ -2177,17 +2180,19 @@ template <int KIND, int R>
 static hipError_t run_focal_r(const Params &P, const Region &rg, const SrcView &s, const DstView *d, int nout,
                               hipStream_t st)
 {
-    const int64_t gx = cdiv(rg.w, kTileX);
-    for (int64_t yb = 0; yb < cdiv(rg.h, kTileY); yb += kMaxGridY) {
+    // A 1-D grid of tiles (xcd_tile numbers them), at most kMaxTileBlocks
+    // workgroups per launch.
+    const int64_t gx = cdiv(rg.w, kTileX), step = std::max<int64_t>(1, kMaxTileBlocks / gx);
+    for (int64_t yb = 0; yb < cdiv(rg.h, kTileY); yb += step) {
         Region sub = rg;
         sub.oy = rg.oy + yb * kTileY;
-        sub.h = min(rg.h - yb * kTileY, kMaxGridY * kTileY);
+        sub.h = min(rg.h - yb * kTileY, step * kTileY);
         DstSet e;
         for (int o = 0; o < kMaxOutputs; ++o) {
             e.v[o] = d[o < nout ? o : 0];
             e.v[o].p += yb * kTileY * e.v[o].stride;
         }
-        dim3 grid((unsigned)gx, (unsigned)cdiv(sub.h, kTileY));
+        const dim3 grid((unsigned)(gx * cdiv(sub.h, kTileY)));
         hipLaunchKernelGGL((focal_tile<KIND, R>), grid, dim3(kTileThreads), 0, st, P, sub, s, e);
     }
     return hipGetLastError();
@@ -2218,15 +2223,15 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
     hipError_t e = scratch_alloc_async((void **)&tab, sizeof(bf16x8) * 2 * 64 * N, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mfma_toeplitz<R>, dim3(N), dim3(64), 0, st, P, tab);
-    const int64_t gx = cdiv(rg.w, kMfmaTile);
-    for (int64_t yb = 0; yb < cdiv(rg.h, kMfmaTile); yb += kMaxGridY) {
+    const int64_t gx = cdiv(rg.w, kMfmaTile), step = std::max<int64_t>(1, kMaxTileBlocks / gx);
+    for (int64_t yb = 0; yb < cdiv(rg.h, kMfmaTile); yb += step) {
         Region sub = rg;
         sub.oy = rg.oy + yb * kMfmaTile;
-        sub.h = min(rg.h - yb * kMfmaTile, kMaxGridY * kMfmaTile);
+        sub.h = min(rg.h - yb * kMfmaTile, step * kMfmaTile);
         DstView e0 = d[0];
         e0.p += yb * kMfmaTile * e0.stride;
-        hipLaunchKernelGGL(correlate_mfma<R>, dim3((unsigned)gx, (unsigned)cdiv(sub.h, kMfmaTile)), dim3(256), 0, st,
-                           P, sub, s, e0, tab);
+        hipLaunchKernelGGL(correlate_mfma<R>, dim3((unsigned)(gx * cdiv(sub.h, kMfmaTile))), dim3(256), 0, st, P,
+                           sub, s, e0, tab);
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(tab, st);

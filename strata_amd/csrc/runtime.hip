@@ -1487,8 +1487,12 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     hipEvent_t done[kSlots] = {nullptr, nullptr};
     auto buf = [&](int slot, int i) -> Buf & { return bufs[slot * nbuf + i]; };
     auto free_bufs = [&]() {
-        // Pinned buffers go back to the cache only once no copy uses them.
-        (void)hipStreamSynchronize(st);
+        // Pinned buffers go back to the cache only once no copy uses them;
+        // device buffers are freed in stream order. A call that took no
+        // pinned buffer (resident operands) does not wait.
+        bool pinned = false;
+        for (auto &b : bufs) pinned = pinned || b.hdata || b.hmask;
+        if (pinned) (void)hipStreamSynchronize(st);
         for (auto &b : bufs) {
             PinnedCache::get().release(b.hdata);
             PinnedCache::get().release(b.hmask);
