@@ -64,8 +64,11 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=10, help="timed runs of the batch")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival time per QPS level of the serving sweep (0: skip it)")
-    ap.add_argument("--serve-qps", type=str, default="100,200,500,1000,2000,5000",
+    ap.add_argument("--serve-qps", type=str, default="100,200,500,1000,2000,5000,10000,20000,50000,100000",
                     help="QPS levels of the serving sweep")
+    ap.add_argument("--serve-burst", type=int, default=2048,
+                    help="requests of the closed-loop serving run (all arriving at once) and of the per-tile "
+                         "comparison on the same request mix (0: skip both)")
     ap.add_argument("--serve-rows", type=int, default=0,
                     help="rows per sweep unit when serving (0: chosen per request; short units run a "
                          "small request's rows in parallel)")
@@ -922,12 +925,86 @@ def serving_bench(args, dem, N, log):
             log(f"bench: serve {qps:.0f} QPS: p50 {lv['p50_ms']} ms, p99 {lv['p99_ms']} ms, "
                 f"{lv['achieved_rps']} req/s, mean batch {lv['mean_batch']}, queue {lv['mean_queue_ms']} ms, "
                 f"service {lv['mean_service_ms']} ms")
+        burst = None
+        if args.serve_burst > 0:
+            # Closed loop past the knee: every request of the mix arrives at
+            # once, so the server runs at its own pace; its throughput is
+            # the served form's capacity.
+            bw, _ = poisson_schedule(1000.0, args.serve_burst, N, N, seed=300)
+            zero = np.zeros(len(bw))
+            srv.replay(bw, zero)  # warm-up of the same shape
+            before = srv.stats()
+            lat = srv.replay(bw, zero)
+            after = srv.stats()
+            span = float(lat.max()) / 1e3
+            batches = after["batches"] - before["batches"]
+            cells = int((bw[:, 2] * bw[:, 3]).sum())
+            burst = {"requests": len(bw), "cells": cells, "makespan_ms": round(span * 1e3, 3),
+                     "tiles_per_s": round(len(bw) / span, 1), "gcells_s": round(cells / span / 1e9, 2),
+                     "batches": batches, "mean_batch": round(len(bw) / max(1, batches), 2),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                     "admission_waits": after["admission_waits"] - before["admission_waits"]}
+            log(f"bench: serve closed loop: {burst}")
         st = srv.stats()
-    return {"workload": f"open-loop Poisson arrivals; windows 80% 512², 15% 2048², 5% 4096² of the resident "
-                        f"{N}x{N} DEM; chain Slope (CellSize 30, degrees) -> Lookup (0,5,10,20,30,40 -> "
-                        f"1,1.1,1.3,1.9,3,4.5) -> Reclass (1.2,1.6,2.2,3 -> 1..5) fused in one pass",
-            "slots": 4, "hbm_budget_bytes": budget, "max_batch": 256, "rows": args.serve_rows, "levels": levels,
-            "peak_result_bytes": st["peak_bytes"], "max_in_flight": st["max_in_flight"]}
+    res = {"workload": f"open-loop Poisson arrivals; windows 80% 512², 15% 2048², 5% 4096² of the resident "
+                       f"{N}x{N} DEM; chain Slope (CellSize 30, degrees) -> Lookup (0,5,10,20,30,40 -> "
+                       f"1,1.1,1.3,1.9,3,4.5) -> Reclass (1.2,1.6,2.2,3 -> 1..5) fused in one pass",
+           "slots": 4, "hbm_budget_bytes": budget, "max_batch": 256, "rows": args.serve_rows, "levels": levels,
+           "peak_result_bytes": st["peak_bytes"], "max_in_flight": st["max_in_flight"]}
+    if burst is not None:
+        res["closed_loop"] = burst
+        res["per_tile"] = serving_per_tile(bw, dem, N, (slope, lookup, reclass), log)
+        res["served_over_per_tile"] = round(burst["tiles_per_s"] / res["per_tile"]["tiles_per_s"], 2)
+        log(f"bench: served / per-tile tiles/s: {res['served_over_per_tile']}")
+    return res
+
+
+def serving_per_tile(windows, dem, N, chain, log):
+    """The same requests through the plain per-tile path: for each window,
+    strata_process of Slope, then of Lookup and Reclass over its output in
+    place, launched back to back on one stream (the batch API's per-tile
+    runner; per window this is faster than one strata_process_chain call),
+    each request with its own output raster."""
+    import numpy as np
+    import torch
+
+    from strata_amd.batch import Batch, Task
+    from strata_amd.raster import Float32Raster
+
+    slope, lookup, reclass = chain
+    dev = dem.Data.device
+    w = windows[:, 2].astype(np.int64)
+    h = windows[:, 3].astype(np.int64)
+    cells = int((w * h).sum())
+    out = torch.empty(cells, dtype=torch.float32, device=dev)
+    mask = torch.zeros((cells + 63) // 64 + 1, dtype=torch.int64, device=dev)
+    tasks, off = [], 0
+    for (x, y, wi, hi_) in np.asarray(windows, dtype=np.int64):
+        x, y, wi, hi_ = int(x), int(y), int(wi), int(hi_)
+        start = y * N + x
+        src = Float32Raster(dem.Data[start:start + (hi_ - 1) * N + wi], wi, hi_, N, dem.Valid, start)
+        dst = Float32Raster(out[off:off + wi * hi_], wi, hi_, wi, mask, off)
+        tasks.append(Task(slope, src, dst, (lookup, reclass)))
+        off += wi * hi_
+    b = Batch(tasks)
+    b.run_per_tile()
+    b.sync()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.run_per_tile()
+    b.sync()
+    s = (time.perf_counter() - t) / reps
+    b.close()
+    del out, mask
+    torch.cuda.empty_cache()
+    res = {"requests": len(tasks), "ms": round(s * 1e3, 3), "tiles_per_s": round(len(tasks) / s, 1),
+           "gcells_s": round(cells / s / 1e9, 2),
+           "launch": "strata_process per request and stage (Slope, then Lookup and Reclass in place), back to back "
+                     "on one stream"}
+    log(f"bench: serve per-tile path: {res}")
+    return res
 
 
 def cpu_baseline(args, dem, out, slope, N, log):

@@ -406,8 +406,9 @@ typedef struct strata_tuning {
      * shorter passes (1..4096, default 4096). */
     int32_t fold_stripe_mb;
     /* Cells per unit, in 1024s, of a call that can be cancelled: the work
-     * claimed between two polls of the cancel flag. 0: the library's
-     * choice, an eighth of the raster and at least 4 Mi cells. 0..2^20. */
+     * claimed between two polls of the cancel flag (four units are in
+     * flight). 0: the library's choice, a sixteenth of the raster and at
+     * least 4 Mi cells. 0..2^20. */
     int32_t unit_kcells;
     int32_t reserved[5];
 } strata_tuning;
@@ -506,7 +507,10 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
  * batch's stream and returns. */
 int strata_batch_run(strata_batch *b);
 /* Enqueues the same tasks through the plain path: one strata_process call
- * per task and per point op (the per-tile launch baseline). */
+ * per task and per point op (the per-tile launch baseline; for point ops
+ * this is faster than one strata_process_chain call per task, whose fused
+ * sweep stages its tables and runs the validity pass in launches of their
+ * own). */
 int strata_batch_run_per_tile(strata_batch *b);
 /* Times the batch at each candidate launch shape (rows per unit, blocks
  * per CU), reps runs each, and keeps the fastest; results never depend

@@ -163,15 +163,22 @@ size_t list_bytes(int nops, size_t ntasks)
     return 256 + align256(sizeof(BatchOp) * (size_t)nops) + 3 * align256(sizeof(BatchTask) * ntasks) + 256 * kBatchFamilies;
 }
 
-// Rows per radius-1 unit that split one task into about kLatencyUnits wave
-// units (a power of two in [4, 64]): a small window then runs as many short
-// units at once instead of a few long row chains.
+// Rows per radius-1 unit (128 columns wide) for a whole list: the smallest
+// power of two in [4, 64] that splits the list's radius-1 tasks into at
+// most kLatencyUnits wave units. A lone small window then runs as many
+// short units at once instead of a few long row chains, and a list of many
+// windows (a served batch at saturation) runs long units, whose two halo
+// rows and prologue weigh little.
 constexpr int64_t kLatencyUnits = 8192;
-int adaptive_rows(const BatchTask &t)
+int adaptive_rows(const BatchOp *ops, const BatchTask *tasks, size_t n)
 {
-    const int64_t across = cdiv64(t.rg.w, 128);
     int rows = 4;
-    while (rows < 64 && across * cdiv64(t.rg.h, rows) > kLatencyUnits) rows *= 2;
+    for (; rows < 64; rows *= 2) {
+        int64_t units = 0;
+        for (size_t i = 0; i < n && units <= kLatencyUnits; ++i)
+            if (ops[tasks[i].op].kind == BK_HORN) units += cdiv64(tasks[i].rg.w, 128) * cdiv64(tasks[i].rg.h, rows);
+        if (units <= kLatencyUnits) break;
+    }
     return rows;
 }
 
@@ -195,11 +202,12 @@ ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, con
     memcpy(out + L.ops, ops, sizeof(BatchOp) * (size_t)nops);
     size_t k[kBatchFamilies] = {0, 0, 0};
     int64_t u[kBatchFamilies] = {0, 0, 0};
+    if (rows <= 0) rows = adaptive_rows(ops, tasks, n);
     BatchTask *lt[kBatchFamilies];
     for (int f = 0; f < kBatchFamilies; ++f) lt[f] = reinterpret_cast<BatchTask *>(out + L.tasks[f]);
     for (size_t i = 0; i < n; ++i) {
         BatchTask t = tasks[i];
-        batch_units(ops[t.op], t, rows > 0 ? rows : adaptive_rows(t));
+        batch_units(ops[t.op], t, rows);
         const int64_t data = (int64_t)t.blocks_x * cdiv64(t.rg.h, t.rows);
         const int64_t vrows = t.vgroup ? kVgRows * (64 / (int64_t)t.vgroup) : kVsRows;  // rows per validity unit
         t.vunits = masked[i] ? (int32_t)((int64_t)t.vwaves_x * cdiv64(t.rg.h, vrows)) : 0;

@@ -1845,30 +1845,44 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
 // The point ops of a batch op, applied to a stencil op's output values. The
 // op table is read-only for the launch and read as constant memory, so the
 // sweep's stores do not make the compiler load it again after each one.
+// A batch op's point stages on a sweep's output values (the counting forms
+// of arith.hpp): the x sides read at uniform indices from the op in device
+// memory, the entries picked gathered from the wave's own LDS copy of the
+// op's tables (lds_table's layout, stage after stage).
+constexpr int kWavePostFloats = 2 * kBatchPostTaps;
 struct ChainPost {
     const STRATA_CONST BatchOp *op;
-    int npost;
-    __device__ explicit ChainPost(const BatchOp *o) : op(cp(o)), npost(cp(o)->npost) {}
+    const STRATA_LDS float *lt;
     __device__ void operator()(float &v0, float &v1) const
     {
-        v0 = one(v0);
-        v1 = one(v1);
-    }
-    __device__ float one(float v) const
-    {
-        for (int i = 0; i < npost; ++i) {
-            const STRATA_CONST float *t = op->post_taps[i];
-            if (op->post_op[i] == STRATA_OP_LOOKUP) {
-                const int n = op->post_n[i] / 2;
-                v = lookup_of(v, t, t + n, n);
-            } else {
-                const int nb = (op->post_n[i] - 1) / 2;
-                v = reclass_of(v, t, nb, t + nb);
-            }
+        float v[2] = {v0, v1};
+        int lo = 0;
+        for (int i = 0; i < op->npost; ++i) {
+            const int o = op->post_op[i], c = op->post_n[i];
+            point_values(v, o, op->post_taps[i], c, lt + lo);
+            lo += lds_table_floats(o, c);
         }
-        return v;
+        v0 = v[0];
+        v1 = v[1];
     }
 };
+
+// Copies op's point tables into the wave's LDS copy; every lane of the wave
+// calls it. LDS operations of one wave complete in order, so the wave's
+// later reads see these writes, and writes made after its earlier reads
+// cannot overtake them.
+__device__ __forceinline__ void stage_wave_post(const BatchOp &op, STRATA_LDS float *lt, int lane)
+{
+    const STRATA_CONST BatchOp *o = cp(&op);
+    int lo = 0;
+    for (int i = 0; i < o->npost; ++i) {
+        lds_table(lt + lo, o->post_op[i], o->post_taps[i], o->post_n[i], lane, 64);
+        lo += lds_table_floats(o->post_op[i], o->post_n[i]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 constexpr int kBatchMaxFocalR = 3;
 constexpr int kBatchLdsFloats = focal_lds_floats<FK_MEAN, kBatchMaxFocalR>();
@@ -1878,7 +1892,8 @@ constexpr int kFocalTilesPerUnit = 4;  // focal tiles stacked in one unit
 constexpr int kWaveCols = 128;
 
 template <int OP>
-__device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t, int64_t lu)
+__device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t, int64_t lu,
+                                           const STRATA_LDS float *lt)
 {
     HornArgs a;
 #pragma unroll
@@ -1894,7 +1909,7 @@ __device__ __forceinline__ void batch_horn(const BatchOp &op, const BatchTask &t
     // ones directly, the others row by row at their own parity.
     const bool inside = xb >= t.s.lx0 && xb + kWaveCols <= t.s.lx1 && xb >= t.rg.ox &&
                         xb + kWaveCols <= t.rg.ox + t.rg.w;
-    const ChainPost post(&op);
+    const ChainPost post{cp(&op), lt};
     const uint32_t lane = threadIdx.x & 63;
     if (inside && t.aligned)
         sweep<OP, kSweepPairs>(a, t.rg, t.s, t.d, t.d, xb, y0, y1, lane, post);
@@ -1994,11 +2009,14 @@ __global__ __launch_bounds__(256) void batch_wave_kernel(const BatchHeader *__re
                                                          const BatchOp *__restrict__ ops, unsigned *counter)
 {
     __shared__ unsigned s_claim[2];
+    __shared__ __align__(16) float s_post[4][kWavePostFloats];  // each wave's copy of its op's point tables
     const int64_t nh = hdr[BF_HORN].nunits, nv = hdr[BF_VALIDITY].nunits;
     const int64_t th = hdr[BF_HORN].ntasks, tv = hdr[BF_VALIDITY].ntasks;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int64_t ti = 0, tu0 = 0, tu1 = 0;  // the current task and its units [tu0, tu1) in its list
     bool in_valid = false;            // which list the cached task belongs to
+    int32_t staged = -1;              // the op whose point tables the wave's LDS copy holds
+    STRATA_LDS float *lt = (STRATA_LDS float *)s_post[wv];
     if (threadIdx.x == 0) s_claim[0] = atomicAdd(counter, 4u);
     __syncthreads();
     int slot = 0;
@@ -2020,11 +2038,15 @@ __global__ __launch_bounds__(256) void batch_wave_kernel(const BatchHeader *__re
         const BatchOp &op = ops[t.op];
         const int64_t lu = lu_all - tu0;
         if (!valid) {
+            if (op.npost > 0 && t.op != staged) {
+                stage_wave_post(op, lt, (int)(threadIdx.x & 63));
+                staged = t.op;
+            }
             switch (op.op) {
-            case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu); break;
-            case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu); break;
-            case STRATA_OP_HILLSHADE: batch_horn<STRATA_OP_HILLSHADE>(op, t, lu); break;
-            default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu); break;
+            case STRATA_OP_SLOPE: batch_horn<STRATA_OP_SLOPE>(op, t, lu, lt); break;
+            case STRATA_OP_ASPECT: batch_horn<STRATA_OP_ASPECT>(op, t, lu, lt); break;
+            case STRATA_OP_HILLSHADE: batch_horn<STRATA_OP_HILLSHADE>(op, t, lu, lt); break;
+            default: batch_horn<STRATA_OP_CURVATURE>(op, t, lu, lt); break;
             }
         } else {
             batch_validity(op, t, lu);

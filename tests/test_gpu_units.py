@@ -1,9 +1,9 @@
 """strata_process over host operands and with a live context.
 
 - Host, device and mixed operands give the oracle's bits; a cancellable
-  call runs as units of the plan claimed in order (two in flight), here at
-  units of 1024 cells that split rows between units and share mask words
-  across them.
+  call runs as units of the plan claimed in order (four in flight over two
+  streams), here at units of 1024 cells that split rows between units and
+  share mask words across them.
 - A context done before the call writes nothing, host operands included
   (internal/exec/doc.go:165-178: "A context that is done before the call
   writes nothing").
