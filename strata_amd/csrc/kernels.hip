@@ -1249,10 +1249,14 @@ __device__ __forceinline__ void split_bf16(float v, __bf16 &h, __bf16 &l)
     l = (__bf16)(v - (float)t);
 }
 
+// bf16 per staged row: the A operands read columns 0..79 (a wave's 16 + 2R
+// <= 32 columns at 16-column steps). At 80 (40 dwords) the sixteen rows one
+// operand read touches fall in distinct LDS banks; 88 cost bank conflicts
+// on every read (half the LDS's busy cycles).
 template <int R>
-constexpr int mfma_pitch()  // bf16 per staged row: at least 48 + 32 columns, a multiple of 8
+constexpr int mfma_pitch()
 {
-    return ((kMfmaTile + 2 * R + 7) / 8) * 8 + 8;
+    return 80;
 }
 
 // Exact FP32 correlate of one cell, in the reference's order; also reports
@@ -1300,12 +1304,52 @@ __global__ __launch_bounds__(64) void mfma_toeplitz(const Params P, bf16x8 *tab)
     gp(tab)[(i * 64 + lane) * 2 + 1] = bl;
 }
 
+// Two staged values as their bf16 high parts (hp) and low parts (lp),
+// packed two to a word (the first in the low half), for finite values whose
+// magnitude is below the largest bf16: the high part is the value rounded
+// to bf16, the low part the rest rounded to bf16.
+__device__ __forceinline__ void split_pair_fast(float a, float b, uint32_t &hp, uint32_t &lp)
+{
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 h{(__bf16)a, (__bf16)b};
+    hp = __builtin_bit_cast(uint32_t, h);
+    const float ha = __builtin_bit_cast(float, hp << 16), hb = __builtin_bit_cast(float, hp & 0xffff0000u);
+    const bf16x2 l{(__bf16)(a - ha), (__bf16)(b - hb)};
+    lp = __builtin_bit_cast(uint32_t, l);
+}
+// The same for any two values: a non-finite value is staged as 0 (and
+// reported), a finite one above the bf16 range through split_bf16.
+__device__ __forceinline__ void split_pair_any(float a, float b, uint32_t &hp, uint32_t &lp, bool &any)
+{
+    if (!__builtin_isfinite(a)) {
+        any = true;
+        a = 0.0f;
+    }
+    if (!__builtin_isfinite(b)) {
+        any = true;
+        b = 0.0f;
+    }
+    __bf16 ha, la, hb, lb;
+    split_bf16(a, ha, la);
+    split_bf16(b, hb, lb);
+    hp = (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+    lp = (uint32_t)__builtin_bit_cast(uint16_t, la) | ((uint32_t)__builtin_bit_cast(uint16_t, lb) << 16);
+}
+// Below the largest bf16 (0x7f7f), so rounding to bf16 stays finite.
+constexpr float kBf16Safe = 3.38e38f;
+
 template <int R>
 __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg, SrcView s, DstView d0,
                                                       const bf16x8 *__restrict__ tab)
 {
     constexpr int N = 2 * R + 1;
     constexpr int LH = kMfmaTile + 2 * R, LW = kMfmaTile + 2 * R, PITCH = mfma_pitch<R>();
+    // Staged columns: the A operands read columns 0..79 of a staged row;
+    // those at LW and past it meet zeros of T_i and are staged as 0.
+    constexpr int PPR = 40;  // column pairs per staged row
+    constexpr int NP = LH * PPR;
+    constexpr int KST = (NP + 255) / 256;
+    static_assert(2 * PPR <= PITCH, "staged pairs fit the LDS row");
     __shared__ __attribute__((aligned(16))) __bf16 hi[LH * PITCH];
     __shared__ __attribute__((aligned(16))) __bf16 lo[LH * PITCH];
     __shared__ float wts[N * N];
@@ -1317,56 +1361,81 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     if (tid == 0) nonfinite = 0;
     for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
     __syncthreads();
-    // Staging in pairs of columns: every load of the thread is issued before
-    // any split (the trip count is a compile-time constant), a tile whose
-    // staged window lies inside the buffer takes unchecked loads, and each
-    // pair's high and low parts go to LDS as one 32-bit store each.
+    // Staging in pairs of columns, every load of the thread issued before
+    // any split (the trip count is a compile-time constant). A tile whose
+    // staged window lies inside the buffer, with row offsets that fit 32
+    // bits, takes unchecked loads at 32-bit byte offsets from one uniform
+    // base. Pairs are split two values at a time; a pass whose values are
+    // all finite and inside the bf16 range (a wave-uniform test) takes the
+    // short split. Each pair's high and low parts go to LDS as one 32-bit
+    // store each.
     {
-        constexpr int PPR = PITCH / 2;  // pairs per staged row
-        constexpr int NP = LH * PPR;
-        constexpr int KST = (NP + 255) / 256;
         const int64_t bx = tx0 - R, by = ty0 - R;
-        const bool inside = bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1;
+        const bool inside = bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1 &&
+                            (int64_t)(LH - 1) * s.stride + LW < ((int64_t)1 << 29);
         float va[KST], vb[KST];
+        // The thread's first pair, then 256 pairs on: 6 rows and 16 pairs.
+        int row = tid / PPR, pc = tid - row * PPR;
         if (inside) {
-            const STRATA_GLOBAL float *base = gp(s.p) + (by - s.sy) * s.stride + (bx - s.sx);
+            const char *base = reinterpret_cast<const char *>(s.p + ((by - s.sy) * s.stride + (bx - s.sx)));
+            const uint32_t st4 = (uint32_t)s.stride * 4u;
+            uint32_t off = (uint32_t)row * st4 + 8u * (uint32_t)pc;
 #pragma unroll
             for (int k = 0; k < KST; ++k) {
-                const int pi = min(tid + k * 256, NP - 1);
-                const int row = pi / PPR, col = 2 * (pi - row * PPR);
-                const STRATA_GLOBAL float *rp = base + (int64_t)row * s.stride;
-                va[k] = col < LW ? rp[col] : 0.0f;
-                vb[k] = col + 1 < LW ? rp[col + 1] : 0.0f;
+                const int col = 2 * pc;
+                // Past the last row or column the load is clamped to the
+                // window's first cell and its value replaced by 0.
+                const bool ok0 = row < LH && col < LW, ok1 = row < LH && col + 1 < LW;
+                const float a = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok0 ? off : 0u));
+                const float b = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok1 ? off + 4u : 0u));
+                va[k] = ok0 ? a : 0.0f;
+                vb[k] = ok1 ? b : 0.0f;
+                pc += 16;
+                row += 6;
+                off += 6u * st4 + 128u;
+                if (pc >= PPR) {
+                    pc -= PPR;
+                    row += 1;
+                    off += st4 - 8u * PPR;
+                }
             }
         } else {
 #pragma unroll
             for (int k = 0; k < KST; ++k) {
-                const int pi = min(tid + k * 256, NP - 1);
-                const int row = pi / PPR, col = 2 * (pi - row * PPR);
-                va[k] = col < LW ? load_cell(s, bx + col, by + row) : 0.0f;
-                vb[k] = col + 1 < LW ? load_cell(s, bx + col + 1, by + row) : 0.0f;
+                const int col = 2 * pc;
+                const bool ok = row < LH;
+                va[k] = ok && col < LW ? load_cell(s, bx + col, by + row) : 0.0f;
+                vb[k] = ok && col + 1 < LW ? load_cell(s, bx + col + 1, by + row) : 0.0f;
+                pc += 16;
+                row += 6;
+                if (pc >= PPR) {
+                    pc -= PPR;
+                    row += 1;
+                }
             }
         }
         bool any = false;
         uint32_t *hi32 = reinterpret_cast<uint32_t *>(hi), *lo32 = reinterpret_cast<uint32_t *>(lo);
+        row = tid / PPR;
+        pc = tid - row * PPR;
 #pragma unroll
         for (int k = 0; k < KST; ++k) {
-            const int pi = tid + k * 256;
-            if (pi < NP) {
-                float a = va[k], b = vb[k];
-                if (!__builtin_isfinite(a)) {
-                    any = true;
-                    a = 0.0f;
-                }
-                if (!__builtin_isfinite(b)) {
-                    any = true;
-                    b = 0.0f;
-                }
-                __bf16 ha, la, hb, lb;
-                split_bf16(a, ha, la);
-                split_bf16(b, hb, lb);
-                hi32[pi] = (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
-                lo32[pi] = (uint32_t)__builtin_bit_cast(uint16_t, la) | ((uint32_t)__builtin_bit_cast(uint16_t, lb) << 16);
+            const bool in = row < LH;
+            const bool plain = __builtin_fabsf(va[k]) < kBf16Safe && __builtin_fabsf(vb[k]) < kBf16Safe;
+            uint32_t hp, lp;
+            if (__builtin_amdgcn_ballot_w64(in && !plain) == 0)
+                split_pair_fast(va[k], vb[k], hp, lp);
+            else
+                split_pair_any(va[k], vb[k], hp, lp, any);
+            if (in) {
+                hi32[row * (PITCH / 2) + pc] = hp;
+                lo32[row * (PITCH / 2) + pc] = lp;
+            }
+            pc += 16;
+            row += 6;
+            if (pc >= PPR) {
+                pc -= PPR;
+                row += 1;
             }
         }
         if (any) nonfinite = 1;
@@ -1378,26 +1447,54 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     f32x4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Not unrolled: unrolled, every weight row's operands would be held in
-    // registers at once.
+    // Each block accumulates hi*hi, hi*lo, lo*hi per weight row, in that
+    // order; the four blocks' products interleave. Weight row i + 1's
+    // operands load while row i's products run. Not unrolled: unrolled,
+    // every weight row's operands would be held in registers at once.
+    const STRATA_GLOBAL bf16x8 *tp = gp(tab) + lane * 2;
+    bf16x8 bh = tp[0], bl = tp[1];
+    const __bf16 *ahp = &hi[(n)*PITCH + c0 + 8 * kg];
+    const __bf16 *alp = &lo[(n)*PITCH + c0 + 8 * kg];
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
-        const bf16x8 bh = gp(tab)[(i * 64 + lane) * 2], bl = gp(tab)[(i * 64 + lane) * 2 + 1];
+        const int ni = i + 1 < N ? i + 1 : i;
+        const bf16x8 nbh = tp[ni * 128], nbl = tp[ni * 128 + 1];
+        bf16x8 ah[4], al[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int at = (16 * b + n + i) * PITCH + c0 + 8 * kg;  // A: row m = n, columns 8kg..8kg+7
-            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&hi[at]);
-            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&lo[at]);
-            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[b], 0, 0, 0);
-            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[b], 0, 0, 0);
-            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[b], 0, 0, 0);
+            ah[b] = *reinterpret_cast<const bf16x8 *>(ahp + (16 * b + i) * PITCH);  // A: rows 16b+i.., columns 8kg..
+            al[b] = *reinterpret_cast<const bf16x8 *>(alp + (16 * b + i) * PITCH);
         }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b], bh, acc[b], 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b], bl, acc[b], 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b], bh, acc[b], 0, 0, 0);
+        bh = nbh;
+        bl = nbl;
     }
 
     const int64_t x = tx0 + c0 + n;
+    const bool fix = nonfinite != 0;
+    // A whole tile inside the region and the interior, with nothing to
+    // recompute, stores without conditions at 32-bit offsets.
+    const bool whole = !fix && tx0 + kMfmaTile <= rg.ox + rg.w && ty0 + kMfmaTile <= rg.oy + rg.h && tx0 >= R &&
+                       tx0 + kMfmaTile <= rg.W - R && ty0 >= R && ty0 + kMfmaTile <= rg.H - R &&
+                       (int64_t)kMfmaTile * d0.stride < ((int64_t)1 << 29);
+    if (whole) {
+        char *ob = reinterpret_cast<char *>(d0.p + ((ty0 - rg.oy) * d0.stride + (tx0 - rg.ox)));
+        const uint32_t st4 = (uint32_t)d0.stride * 4u;
+        const uint32_t o0 = (uint32_t)(4 * kg) * st4 + 4u * (uint32_t)(c0 + n);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                *reinterpret_cast<STRATA_GLOBAL float *>(gp(ob) + o0 + (uint32_t)(16 * b + e) * st4) = acc[b][e];
+        return;
+    }
     const bool col_in = x < rg.ox + rg.w;
     const bool col_interior = x >= R && x < rg.W - R;
-    const bool fix = nonfinite != 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
