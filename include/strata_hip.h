@@ -410,7 +410,11 @@ typedef struct strata_tuning {
      * flight). 0: the library's choice, a sixteenth of the raster and at
      * least 4 Mi cells. 0..2^20. */
     int32_t unit_kcells;
-    int32_t reserved[5];
+    /* For tests of cancellation: when > 0, a strata_process call with a
+     * cancel flag claims at most this many units and then ends as if its
+     * context had been cancelled (no effect on calls without a flag). 0: off. */
+    int32_t stop_after_units;
+    int32_t reserved[4];
 } strata_tuning;
 
 int strata_get_tuning(strata_tuning *t);

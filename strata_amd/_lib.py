@@ -104,7 +104,7 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("sweep_rows", c_int32), ("fold_stripe_mb", c_int32), ("unit_kcells", c_int32),
-                ("reserved", c_int32 * 5)]
+                ("stop_after_units", c_int32), ("reserved", c_int32 * 4)]
 
 
 READ_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,

@@ -2200,7 +2200,7 @@ constexpr int64_t kMaxGridY = 65535;
 constexpr int64_t kMaxTileBlocks = int64_t(1) << 23;
 
 // Launch-shape tuning (never changes a result).
-static Tuning g_tuning{16, 4096, 0, {0, 0, 0, 0, 0}};
+static Tuning g_tuning{16, 4096, 0, 0, {0, 0, 0, 0}};
 static int sweep_rows() { return g_tuning.sweep_rows; }
 
 Tuning get_tuning() { return g_tuning; }
@@ -2209,7 +2209,7 @@ bool set_tuning(const Tuning &t)
 {
     if (t.sweep_rows < 4 || t.sweep_rows > 4096) return false;
     if (t.fold_stripe_mb < 1 || t.fold_stripe_mb > 4096) return false;
-    if (t.unit_kcells < 0 || t.unit_kcells > (1 << 20)) return false;
+    if (t.unit_kcells < 0 || t.unit_kcells > (1 << 20) || t.stop_after_units < 0) return false;
     g_tuning = t;
     return true;
 }

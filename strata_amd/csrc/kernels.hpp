@@ -98,7 +98,8 @@ struct Tuning {
     int32_t sweep_rows;      // rows per block of the radius-1 sweep
     int32_t fold_stripe_mb;  // output MiB per step of a resident fold-only pass
     int32_t unit_kcells;     // cells per unit of a cancellable call, in 1024s (0: the default)
-    int32_t reserved[5];
+    int32_t stop_after_units;  // tests: a cancellable call stops after this many units (0: off)
+    int32_t reserved[4];
 };
 Tuning get_tuning();
 bool set_tuning(const Tuning &t);

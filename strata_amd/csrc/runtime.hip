@@ -833,6 +833,7 @@ int strata_get_tuning(strata_tuning *t)
     t->sweep_rows = g.sweep_rows;
     t->fold_stripe_mb = g.fold_stripe_mb;
     t->unit_kcells = g.unit_kcells;
+    t->stop_after_units = g.stop_after_units;
     return STRATA_OK;
 }
 
@@ -843,11 +844,12 @@ int strata_set_tuning(const strata_tuning *t)
     g.sweep_rows = t->sweep_rows;
     g.fold_stripe_mb = t->fold_stripe_mb;
     g.unit_kcells = t->unit_kcells;
+    g.stop_after_units = t->stop_after_units;
     if (!set_tuning(g))
         return fail(STRATA_EINVAL,
-                    "strata: sweep_rows %d outside [4, 4096], fold_stripe_mb %d outside [1, 4096] or unit_kcells %d "
-                    "outside [0, 1048576]",
-                    t->sweep_rows, t->fold_stripe_mb, t->unit_kcells);
+                    "strata: sweep_rows %d outside [4, 4096], fold_stripe_mb %d outside [1, 4096], unit_kcells %d "
+                    "outside [0, 1048576] or stop_after_units %d negative",
+                    t->sweep_rows, t->fold_stripe_mb, t->unit_kcells, t->stop_after_units);
     g_err.clear();
     return STRATA_OK;
 }
@@ -1257,7 +1259,8 @@ int run_units(const Job &J, const Operands &op, const Plan &p, const strata_opti
     for (int64_t i = 0; i < p.bands; ++n) {
         const int k = (int)(n % kUnitsAhead);
         if ((rc = finish(k))) return rc;
-        if (cancelled(opts)) {
+        const int32_t limit = get_tuning().stop_after_units;
+        if (cancelled(opts) || (limit > 0 && n >= limit)) {
             status = STRATA_CANCELLED;
             break;
         }

@@ -13,7 +13,10 @@
   watcher thread reads a device output cell of an early unit from its own
   stream and cancels as soon as it sees the cell written, so the cancel
   lands mid-run; Gradient's second output in host memory shows that a host
-  output keeps the same prefix.
+  output keeps the same prefix. The same holds, deterministically, when the
+  library's test knob stops a call after a set number of units, and when
+  another thread cancels a call while it runs (whether it lands mid-run or
+  after the end).
 """
 import contextlib
 import threading
@@ -45,6 +48,21 @@ def unit_kcells(n):
         yield
     finally:
         t.unit_kcells = old
+        assert L.strata_set_tuning(t) == 0
+
+
+@contextlib.contextmanager
+def stop_after_units(n):
+    L = _lib.lib()
+    t = _lib.Tuning()
+    assert L.strata_get_tuning(t) == 0
+    old = t.stop_after_units
+    t.stop_after_units = n
+    assert L.strata_set_tuning(t) == 0, _lib.last_error()
+    try:
+        yield
+    finally:
+        t.stop_after_units = old
         assert L.strata_set_tuning(t) == 0
 
 
@@ -233,3 +251,74 @@ def test_cancellable_device_call_in_many_units(gpu, plan):
         assert err is None
         assert_same_parents(host_copy(dbufs), want, f"{name} in units, plan {plan}")
         assert st == engine.PlanStats(k, 1531, 1207, Options(TileWidth=plan[0], TileHeight=plan[1]))
+
+
+@pytest.mark.parametrize("plan", [(0, 0), (300, 100), (0, 37)])
+@pytest.mark.parametrize("where", ["device", "mixed-host-src"])
+def test_stop_after_units_leaves_band_prefix(gpu, plan, where):
+    """The call stops after three units (the library's test knob, the same
+    path as a flag seen set before the fourth claim): exactly the bands of
+    those units are written and counted."""
+    import torch
+    k = K["slope"] if where == "device" else K["gradient"]
+    W, H = 1500, 1300
+    tw, th = plan
+    sc = build_scene("windows", k, 19, W, H)
+    host, want = sc.materialize(None)
+    oracle.process(k, [host[f"d{o}"] for o in range(k.nout)], [host["s"]])
+    sw = "host" if where == "mixed-host-src" else "device"
+    dw = ["device"] if where == "device" else ["device", "host"]
+    src, dsts, bufs = mixed(sc, gpu, sw, dw, k.nout)
+    before = host_copy(bufs)
+    st = engine.Stats()
+    with unit_kcells(16), stop_after_units(3):
+        err = ProcessN(Context(), dsts, [src], k, Options(TileWidth=tw, TileHeight=th, Stats=st))
+    torch.cuda.synchronize()
+    total = len(plan_bands(W, H, tw, th))
+    assert isinstance(err, engine.Canceled)
+    assert 0 < st.Bands < total
+    assert st == prefix_stats(k, W, H, tw, th, st.Bands)
+    exp = expected_prefix(sc, k, W, H, tw, th, st.Bands, before, want)
+    assert_same_parents(host_copy(bufs), exp, f"stopped after three units, {where} {plan}")
+
+
+@pytest.mark.parametrize("name", ["slope", "mean3"])
+def test_cancel_from_another_thread(gpu, name):
+    """A call cancelled from another thread while it runs either finishes
+    first (every band written) or ends cancelled with exactly a prefix of
+    the plan's bands written, the rest untouched, and that prefix's Stats.
+    Both are the reference's contract; which one happens depends on
+    timing."""
+    import torch
+    k = K[name]
+    W, H = 3001, 2003
+    sc = build_scene("windows", k, 29, W, H)
+    host, want = sc.materialize(None)
+    oracle.process(k, [host[f"d{o}"] for o in range(k.nout)], [host["s"]])
+    src, dsts, bufs = mixed(sc, gpu, "device", ["device"], k.nout)
+    before = host_copy(bufs)
+    ctx = Context()
+    started = threading.Event()
+
+    def cancel_soon():
+        started.wait()
+        ctx.cancel()
+
+    th_c = threading.Thread(target=cancel_soon)
+    th_c.start()
+    st = engine.Stats()
+    with unit_kcells(64):
+        started.set()
+        err = ProcessN(ctx, dsts, [src], k, Options(Stats=st))
+    th_c.join()
+    torch.cuda.synchronize()
+    total = len(plan_bands(W, H, 0, 0))
+    print(f"{name}: {'cancelled' if err else 'finished'} with {st.Bands} of {total} bands", flush=True)
+    if err is None:
+        assert st.Bands == total
+    else:
+        assert isinstance(err, engine.Canceled)
+        assert 0 <= st.Bands < total
+    assert st == prefix_stats(k, W, H, 0, 0, st.Bands)
+    exp = expected_prefix(sc, k, W, H, 0, 0, st.Bands, before, want)
+    assert_same_parents(host_copy(bufs), exp, f"{name} cancelled from a thread")
