@@ -662,6 +662,12 @@ struct Staged {
     }
 };
 
+// Pageable host memory to and from device memory through pinned buffers,
+// overlapping the host copies with the DMA (defined with the IO pool).
+int host_to_device(void *dev, const void *host, size_t bytes, hipStream_t st);
+int device_to_host_2d(void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width, size_t rows,
+                      hipStream_t st);
+
 int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
 {
     hipMemPool_t pool = nullptr;
@@ -673,15 +679,13 @@ int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
     const int64_t n = data_len(h);
     HIP_TRY(hipMallocFromPoolAsync(&s.dbuf, (size_t)n * sizeof(float), pool, st), "staging data");
     s.dev.data = (float *)s.dbuf;
-    if (copy_data)
-        HIP_TRY(hipMemcpyAsync(s.dbuf, h.data, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st), "copy in");
+    if (copy_data && (rc = host_to_device(s.dbuf, h.data, (size_t)n * sizeof(float), st))) return rc;
     if (h.valid) {
         s.w0 = h.valid_offset >> 6;
         const int64_t w1 = (h.valid_offset + n - 1) >> 6;
         s.nwords = w1 - s.w0 + 1;
         HIP_TRY(hipMallocFromPoolAsync((void **)&s.dmask, (size_t)s.nwords * 8, pool, st), "staging mask");
-        HIP_TRY(hipMemcpyAsync(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, hipMemcpyHostToDevice, st),
-                "copy mask in");
+        if ((rc = host_to_device(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, st))) return rc;
         s.dev.valid = s.dmask;
         s.dev.valid_offset = h.valid_offset - 64 * s.w0;
     }
@@ -693,12 +697,12 @@ int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
 // were copied in whole, so bits of other cells go back as they came.
 int stage_out(const strata_raster &h, Staged &s, hipStream_t st)
 {
-    HIP_TRY(hipMemcpy2DAsync(h.data, (size_t)h.stride * 4, s.dbuf, (size_t)h.stride * 4, (size_t)h.width * 4,
-                             (size_t)h.height, hipMemcpyDeviceToHost, st),
-            "copy out");
-    if (h.valid)
-        HIP_TRY(hipMemcpyAsync(h.valid + s.w0, s.dmask, (size_t)s.nwords * 8, hipMemcpyDeviceToHost, st),
-                "copy mask out");
+    int rc = device_to_host_2d(h.data, (size_t)h.stride * 4, s.dbuf, (size_t)h.stride * 4, (size_t)h.width * 4,
+                               (size_t)h.height, st);
+    if (rc) return rc;
+    if (h.valid && (rc = device_to_host_2d(h.valid + s.w0, (size_t)s.nwords * 8, s.dmask, (size_t)s.nwords * 8,
+                                           (size_t)s.nwords * 8, 1, st)))
+        return rc;
     return STRATA_OK;
 }
 
@@ -1143,6 +1147,137 @@ class PinnedCache {
 };
 
 void pinned_trim() { PinnedCache::get().trim(); }
+
+// The pieces pageable copies move through pinned memory in, and the size
+// below which the runtime's own copy is used.
+constexpr size_t kHostPiece = (size_t)32 << 20, kHostDirect = (size_t)4 << 20;
+
+// A copy between pageable and pinned memory split over the IO pool's
+// threads, in parts of at least 2 MiB.
+void pool_memcpy(void *dst, const void *src, size_t n)
+{
+    IoPool &pool = IoPool::get();
+    const int parts = (int)std::max<size_t>(1, std::min<size_t>((size_t)pool.threads(), n >> 21));
+    if (parts == 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    pool.run(parts, [&](int k) {
+        const size_t a = n * (size_t)k / (size_t)parts, b = n * (size_t)(k + 1) / (size_t)parts;
+        memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a);
+    });
+}
+
+// Two pinned pieces and their events, given back to the cache when done.
+struct PinnedPair {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PinnedPair()
+    {
+        for (int k = 0; k < 2; ++k) {
+            if (ev[k]) {
+                (void)hipEventSynchronize(ev[k]);
+                (void)hipEventDestroy(ev[k]);
+            }
+            PinnedCache::get().release(buf[k]);
+        }
+    }
+    int init()
+    {
+        for (int k = 0; k < 2; ++k) {
+            int rc = PinnedCache::get().acquire(kHostPiece, &buf[k]);
+            if (rc) return rc;
+            HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "staging event");
+        }
+        return STRATA_OK;
+    }
+};
+
+// Host to device: the IO pool copies piece i into one pinned buffer while
+// the DMA of piece i-1 runs from the other (the runtime's copy from
+// pageable memory runs the two steps one after the other).
+int host_to_device(void *dev, const void *host, size_t bytes, hipStream_t st)
+{
+    if (bytes < kHostDirect) {
+        HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st), "copy in");
+        return STRATA_OK;
+    }
+    PinnedPair pp;
+    int rc = pp.init();
+    if (rc) return rc;
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kHostPiece, ++i) {
+        const int k = (int)(i & 1);
+        const size_t n = std::min(kHostPiece, bytes - off);
+        if (i >= 2) HIP_TRY(hipEventSynchronize(pp.ev[k]), "waiting for a staging buffer");
+        pool_memcpy(pp.buf[k], static_cast<const char *>(host) + off, n);
+        HIP_TRY(hipMemcpyAsync(static_cast<char *>(dev) + off, pp.buf[k], n, hipMemcpyHostToDevice, st), "copy in");
+        HIP_TRY(hipEventRecord(pp.ev[k], st), "staging event");
+    }
+    return STRATA_OK;
+}
+
+// Device to host, rows of `width` bytes (dpitch apart on the device, hpitch
+// in host memory): whole device rows go to a pinned buffer by DMA, two
+// pieces in flight, and the IO pool copies each piece's rows out while the
+// next piece's DMA runs.
+int device_to_host_2d(void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width, size_t rows,
+                      hipStream_t st)
+{
+    if (rows == 0 || width == 0) return STRATA_OK;
+    const size_t piece = kHostPiece / 4;
+    if (rows == 1 && width > piece) {
+        // One long row: rows of `piece` bytes, then the rest (each call
+        // below has rows of at most `piece` bytes, so it does not split again).
+        const size_t full = width / piece;
+        int rc = full ? device_to_host_2d(host, piece, dev, piece, piece, full, st) : STRATA_OK;
+        if (rc || width == full * piece) return rc;
+        return device_to_host_2d(static_cast<char *>(host) + full * piece, width - full * piece,
+                                 static_cast<const char *>(dev) + full * piece, width - full * piece,
+                                 width - full * piece, 1, st);
+    }
+    if (dpitch * rows < kHostDirect || dpitch > kHostPiece) {
+        HIP_TRY(hipMemcpy2DAsync(host, hpitch, dev, dpitch, width, rows, hipMemcpyDeviceToHost, st), "copy out");
+        HIP_TRY(hipStreamSynchronize(st), "copy out");
+        return STRATA_OK;
+    }
+    PinnedPair pp;
+    int rc = pp.init();
+    if (rc) return rc;
+    const size_t per = kHostPiece / dpitch;  // rows per piece
+    const size_t pieces = (rows + per - 1) / per;
+    auto issue = [&](size_t i) -> int {
+        const int k = (int)(i & 1);
+        const size_t r0 = i * per, nr = std::min(per, rows - r0);
+        // The last row ends at its width: the device buffer may end there.
+        HIP_TRY(hipMemcpyAsync(pp.buf[k], static_cast<const char *>(dev) + r0 * dpitch, (nr - 1) * dpitch + width,
+                               hipMemcpyDeviceToHost, st),
+                "copy out");
+        HIP_TRY(hipEventRecord(pp.ev[k], st), "staging event");
+        return STRATA_OK;
+    };
+    for (size_t i = 0; i < std::min<size_t>(2, pieces); ++i)
+        if ((rc = issue(i))) return rc;
+    for (size_t i = 0; i < pieces; ++i) {
+        const int k = (int)(i & 1);
+        HIP_TRY(hipEventSynchronize(pp.ev[k]), "waiting for a staging buffer");
+        const size_t r0 = i * per, nr = std::min(per, rows - r0);
+        const char *src = static_cast<const char *>(pp.buf[k]);
+        char *dst = static_cast<char *>(host) + r0 * hpitch;
+        if (hpitch == dpitch && width == dpitch)
+            pool_memcpy(dst, src, nr * width);
+        else {
+            IoPool &pool = IoPool::get();
+            const int parts = (int)std::max<size_t>(1, std::min<size_t>((size_t)pool.threads(), nr));
+            pool.run(parts, [&](int q) {
+                for (size_t r = nr * (size_t)q / (size_t)parts; r < nr * (size_t)(q + 1) / (size_t)parts; ++r)
+                    memcpy(dst + r * hpitch, src + r * dpitch, width);
+            });
+        }
+        if (i + 2 < pieces && (rc = issue(i + 2))) return rc;
+    }
+    return STRATA_OK;
+}
 
 // Second streams of cancellable calls, kept per device for reuse (default
 // priority, non-blocking). A call takes one for its duration.
