@@ -344,9 +344,10 @@ def paths_bench(args, dem, out, src, sink, slope, N, dev, log):
       exec.Pipeline call, SlopeChunked-style over the resident DEM, against
       the three calls it replaces, with the outputs compared bit for bit;
     - host memory: the DEM and the output in host (pageable) memory, as a
-      Go caller hands them over, through ProcessN (staged whole) and through
-      ProcessChunked with TileHeight 256 (tiles read and written through the
-      host callbacks), in Gcells/s and in GB/s over PCIe; the output is
+      Go caller hands them over, through ProcessN (staged whole through
+      pinned pieces) and through ProcessChunked with TileHeight 256
+      (MemorySource/MemorySink windows copied by the library into pinned
+      tile buffers), in Gcells/s and in GB/s over PCIe; the output is
       compared with the device-resident result bit for bit."""
     import numpy as np
     import torch
@@ -434,8 +435,8 @@ def paths_bench(args, dem, out, src, sink, slope, N, dev, log):
                     "pcie_gb_s": round(chunk_b / h_chunk / 1e9, 2), "identical": ok_chunk,
                     "tile_height": args.tile_height},
         "what": f"terrain.Slope {N}x{N} with the DEM and the output in host (pageable) memory: ProcessN "
-                f"(whole rasters staged through HBM) and ProcessChunked (MemorySource/MemorySink tiles through "
-                f"the host callbacks)"}
+                f"(whole rasters staged through HBM in pinned pieces) and ProcessChunked (MemorySource/MemorySink "
+                f"windows copied by the library into pinned tile buffers)"}
     log(f"bench: host memory {res['host_memory']}")
     return res
 

@@ -206,8 +206,10 @@ typedef int (*strata_write_fn)(void *user, const float *buf, int64_t w, int64_t 
                                const uint64_t *valid, int64_t valid_offset, int64_t x, int64_t y);
 
 /* A source is either resident (a raster the library reads windows of
- * directly, MemorySource over device memory: engine/source.go:76-107) or
- * a callback reader. */
+ * itself, engine.MemorySource: engine/source.go:76-107; in device memory
+ * read in place, in host memory copied window by window into pinned tile
+ * buffers by the library's IO threads), a raw file, or a callback reader.
+ * A resident sink likewise (engine.MemorySink). */
 /* engine.RawSource / RawSink (engine/raw.go:28-245) over a file the
  * library reads and writes itself: width*height little-endian float32
  * cells, row-major, from byte `offset`. With has_fill a source is Masked
@@ -260,8 +262,8 @@ typedef struct strata_sink {
  * order, each read with its halo clipped to the raster, computed exactly
  * as strata_process computes those cells, and written to every sink.
  * Device-resident sources and sinks form a device tile queue (no copies);
- * callback ones are staged through pinned host buffers with copies
- * overlapped on separate streams. Tiles are claimed in order and finished
+ * host-resident, raw and callback ones are staged through pinned host
+ * buffers, a tile's copies overlapping the previous tile's sink writes. Tiles are claimed in order and finished
  * once claimed, so a cancelled or failed call leaves a prefix of whole
  * tiles (engine/doc.go:188-215). Stats as chunked.go:302-374.
  */

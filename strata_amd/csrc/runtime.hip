@@ -1217,6 +1217,53 @@ int host_to_device(void *dev, const void *host, size_t bytes, hipStream_t st)
     return STRATA_OK;
 }
 
+// Copies n bits from src at bit sb to dst at bit db (LSB-first 64-bit
+// words); dst's other bits stay as they were.
+void copy_bits(uint64_t *dst, int64_t db, const uint64_t *src, int64_t sb, int64_t n)
+{
+    while (n > 0) {
+        const int64_t dw = db >> 6, sw = sb >> 6;
+        const int dbit = (int)(db & 63), sbit = (int)(sb & 63);
+        const int m = (int)std::min<int64_t>(n, 64 - dbit);
+        uint64_t v = src[sw] >> sbit;
+        if (sbit + m > 64) v |= src[sw + 1] << (64 - sbit);
+        const uint64_t mask = m == 64 ? ~0ull : ((1ull << m) - 1);
+        dst[dw] = (dst[dw] & ~(mask << dbit)) | ((v & mask) << dbit);
+        db += m;
+        sb += m;
+        n -= m;
+    }
+}
+
+// A window of a host-memory raster to and from a tile buffer, as
+// engine.MemorySource.ReadWindow and MemorySink.WriteWindow do: cells of w x h
+// at (x, y), rows `stride` cells apart in the buffer, the buffer's bit of its
+// cell (i, j) at voff + j*stride + i. Rows are copied by the IO pool; bits
+// one row after another (rows of the raster may share mask words).
+void host_window(const strata_raster &r, bool write, float *buf, uint64_t *bits, int64_t w, int64_t h,
+                 int64_t stride, int64_t voff, int64_t x, int64_t y)
+{
+    IoPool &pool = IoPool::get();
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)pool.threads(), h, (4 * w * h) >> 21}));
+    pool.run(parts, [&](int k) {
+        for (int64_t j = h * k / parts; j < h * (k + 1) / parts; ++j) {
+            float *rp = r.data + (y + j) * r.stride + x, *bp = buf + j * stride;
+            if (write)
+                memcpy(rp, bp, (size_t)w * 4);
+            else
+                memcpy(bp, rp, (size_t)w * 4);
+        }
+    });
+    if (!bits || !r.valid) return;
+    for (int64_t j = 0; j < h; ++j) {
+        const int64_t rb = r.valid_offset + (y + j) * r.stride + x, bb = voff + j * stride;
+        if (write)
+            copy_bits(r.valid, rb, bits, bb, w);
+        else
+            copy_bits(bits, bb, r.valid, rb, w);
+    }
+}
+
 // Device to host, rows of `width` bytes (dpitch apart on the device, hpitch
 // in host memory): whole device rows go to a pinned buffer by DMA, two
 // pieces in flight, and the IO pool copies each piece's rows out while the
@@ -1627,6 +1674,11 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
     FoldScratch folds[kMaxDst];
     bool fold_first[kMaxDst];
     auto fold_only = [&](int o) { return dst[o].fold && !dst[o].resident && !dst[o].raw && !dst[o].write; };
+    // Resident operands in device memory are read and written in place;
+    // those in host memory go through the tile buffers, copied by the
+    // library itself (as the callbacks of engine.MemorySource/MemorySink).
+    auto dev_src = [&](int i) { return src[i].resident && src[i].resident->mem != STRATA_MEM_HOST; };
+    auto dev_dst = [&](int o) { return dst[o].resident && dst[o].resident->mem != STRATA_MEM_HOST; };
     for (int o = 0; o < ndst; ++o) {
         fold_first[o] = true;
         folds[o].st = st;
@@ -1722,9 +1774,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         Guard guard{free_bufs};
         strata_stats s{};
         bool all_resident = true, any_fold_only = false;
-        for (int i = 0; i < nsrc; ++i) all_resident = all_resident && src[i].resident;
+        for (int i = 0; i < nsrc; ++i) all_resident = all_resident && dev_src(i);
         for (int o = 0; o < ndst; ++o) {
-            all_resident = all_resident && (dst[o].resident || fold_only(o));
+            all_resident = all_resident && (dev_dst(o) || fold_only(o));
             any_fold_only = any_fold_only || fold_only(o);
         }
         if (all_resident && !opts_cancel) {
@@ -1804,9 +1856,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
         }
         for (int slot = 0; slot < kSlots; ++slot) {
             for (int i = 0; i < nsrc; ++i)
-                if (!src[i].resident && (rc = alloc_buf(buf(slot, i), inW, inH, src[i].masked))) return rc;
+                if (!dev_src(i) && (rc = alloc_buf(buf(slot, i), inW, inH, src[i].masked))) return rc;
             for (int o = 0; o < ndst; ++o)
-                if (!dst[o].resident &&
+                if (!dev_dst(o) &&
                     (rc = alloc_buf(buf(slot, nsrc + o), g.tileW, g.tileH, dst[o].masked, !fold_only(o))))
                     return rc;
             HIP_TRY(hipEventCreateWithFlags(&done[slot], hipEventDisableTiming), "tile event");
@@ -1824,9 +1876,12 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             HIP_TRY(hipEventSynchronize(done[p.slot]), "waiting for a tile");
             const int64_t tw = p.tr.x1 - p.tr.x0, th = p.tr.y1 - p.tr.y0;
             for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident || fold_only(o)) continue;
+                if (dev_dst(o) || fold_only(o)) continue;
                 Buf &b = buf(p.slot, nsrc + o);
-                if (dst[o].raw) {
+                if (dst[o].resident) {
+                    host_window(*dst[o].resident, true, b.hdata, dst[o].masked ? b.hmask : nullptr, tw, th, b.stride,
+                                0, p.tr.x0, p.tr.y0);
+                } else if (dst[o].raw) {
                     int e = raw_write(*dst[o].raw, W, b.hdata, tw, th, b.stride, p.tr.x0, p.tr.y0, o);
                     if (e) return e;
                 } else if (dst[o].write(dst[o].user, b.hdata, tw, th, b.stride, b.hmask, 0, p.tr.x0, p.tr.y0) != 0) {
@@ -1854,12 +1909,8 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             // tile was handed off by the previous iteration's finish().
             strata_raster sv{};
             int64_t sx = 0, sy = 0;
-            if (src[0].resident) {
+            if (dev_src(0)) {
                 sv = *src[0].resident;
-                if (sv.mem == STRATA_MEM_HOST) {
-                    status = fail(STRATA_EINVAL, "engine: resident sources must be in device memory");
-                    break;
-                }
                 if (!src[0].masked) sv.valid = nullptr;
             } else {
                 Buf &b = buf(slot, 0);
@@ -1874,6 +1925,15 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                     if (b.dmask)
                         HIP_TRY(launch_raw_validity(RawTile{b.ddata, b.stride, ww, hh, b.dmask}, src[0].raw->fill, st),
                                 "launching the fill validity kernel");
+                } else if (src[0].resident) {
+                    if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
+                    host_window(*src[0].resident, false, b.hdata, b.hmask, ww, hh, b.stride, 0, rx0, ry0);
+                    HIP_TRY(hipMemcpyAsync(b.ddata, b.hdata, (size_t)cells * 4, hipMemcpyHostToDevice, st),
+                            "tile in");
+                    if (b.hmask)
+                        HIP_TRY(hipMemcpyAsync(b.dmask, b.hmask, (size_t)cdiv(cells, 64) * 8, hipMemcpyHostToDevice,
+                                               st),
+                                "tile mask in");
                 } else {
                     if (b.hmask) memset(b.hmask, 0, (size_t)b.words * 8);
                     if (src[0].read(src[0].user, b.hdata, ww, hh, b.stride, b.hmask, 0, rx0, ry0) != 0) {
@@ -1900,11 +1960,9 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
             strata_raster dv[kMaxDst]{};
             Operands op{&sv, sx, sy, {}, {}, {}, ndst, W, H};
             for (int o = 0; o < ndst; ++o) op.dst[o] = &dv[o];
-            bool bad_sink = false;
             for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident) {
+                if (dev_dst(o)) {
                     dv[o] = *dst[o].resident;
-                    if (dv[o].mem == STRATA_MEM_HOST) bad_sink = true;
                     if (!dst[o].masked) dv[o].valid = nullptr;
                 } else {
                     Buf &b = buf(slot, nsrc + o);
@@ -1918,17 +1976,13 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                     op.dy[o] = tr.y0;
                 }
             }
-            if (bad_sink) {
-                status = fail(STRATA_EINVAL, "engine: resident sinks must be in device memory");
-                break;
-            }
             if ((rc = run_rect(J, op, tr, st))) return rc;
             // Folds see the tile before its sink does (graph/sinks.go:32-38).
             for (int o = 0; o < ndst; ++o)
                 if (dst[o].fold && (rc = fold_tile(o, dv[o], op.dx[o], op.dy[o], tr))) return rc;
             // Buffered outputs back to the host: a raw sink's fill first.
             for (int o = 0; o < ndst; ++o) {
-                if (dst[o].resident || fold_only(o)) continue;
+                if (dev_dst(o) || fold_only(o)) continue;
                 Buf &b = buf(slot, nsrc + o);
                 const int64_t cells = (th - 1) * b.stride + tw;
                 if (dst[o].raw && b.dmask)

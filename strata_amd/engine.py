@@ -539,8 +539,27 @@ NewMemorySource = MemorySource
 NewMemorySink = MemorySink
 
 
+def _plain_memory(x) -> bool:
+    """A memory source or sink whose window copies are MemorySource's and
+    MemorySink's own (a subclass that overrides them is a callback)."""
+    if isinstance(x, MemorySource):
+        return type(x).ReadWindow is MemorySource.ReadWindow
+    if isinstance(x, MemorySink):
+        return type(x).WriteWindow is MemorySink.WriteWindow
+    return False
+
+
 def _resident(x) -> Optional[Float32Raster]:
-    if isinstance(x, (MemorySource, MemorySink)) and x.r.on_device:
+    if _plain_memory(x) and x.r.on_device:
+        return x.r
+    return None
+
+
+def _in_memory(x) -> Optional[Float32Raster]:
+    """The raster of a memory source or sink, in device or host memory: the
+    library reads device rasters in place and copies windows of host ones
+    itself, as MemorySource.ReadWindow and MemorySink.WriteWindow do."""
+    if _plain_memory(x):
         return x.r
     return None
 
@@ -996,7 +1015,7 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
         sw, sh = s.Size()
         c = csrc[j]
         c.width, c.height, c.masked = sw, sh, 1 if s.Masked() else 0
-        res = _resident(s)
+        res = _in_memory(s)
         raw = _raw(s)
         if res is not None:
             rc_ = res.to_c()
@@ -1022,7 +1041,7 @@ def ProcessChunked(ctx: Optional[Context], dst: List[RasterSink], src: List[Rast
         d = targets[j]
         if d is None:
             continue
-        res = _resident(d)
+        res = _in_memory(d)
         raw = _raw(d)
         if res is not None:
             rc_ = res.to_c()
