@@ -77,7 +77,7 @@ def parse():
     ap.add_argument("--focal-steps", type=int, default=5, help="timed runs of each focal op")
     ap.add_argument("--reduce-steps", type=int, default=10,
                     help="timed runs of each reduction over the DEM (0: skip)")
-    ap.add_argument("--raw-size", type=int, default=8192,
+    ap.add_argument("--raw-size", type=int, default=20000,
                     help="side of the file-to-file SlopeChunked run (raw float32 files; 0: skip)")
     ap.add_argument("--no-paths", dest="paths", action="store_false",
                     help="skip the cancellable, chain and host-memory Slope runs")
@@ -483,7 +483,10 @@ def raw_bench(args, dev, log):
         el = min(runs)
         res = {"size": N, "tile_height": args.tile_height, "s": round(el, 4),
                "gcells_s": round(N * N / el / 1e9, 3), "file_gb_s": round(8 * N * N / el / 1e9, 3),
-               "runs_s": [round(r, 4) for r in runs]}
+               "runs_s": [round(r, 4) for r in runs],
+               # benchmarks/chunked/RESULTS.md:346 (Slope 20000^2, Chunked strips 256, raw
+               # file to file, 12 workers on a Ryzen 9 3900X): another machine's figure.
+               "reference_published_gcells_s": 0.832 if N == 20000 else None}
     log(f"bench: file-to-file {res}")
     return res
 
