@@ -4,6 +4,11 @@ One hipcc invocation over the HIP sources in strata_amd/csrc. Flags:
   -ffp-contract=off   no fused multiply-add: every float32 operation rounds
                       once, in the reference's order
   packed-fp32-ops off the device pass must not emit v_pk_{add,mul,fma}_f32
+  amdgpu-mfma-vgpr-form
+                      matrix-core accumulators in ordinary VGPRs: with the
+                      AGPR form the MFMA Correlate's loop renamed its four
+                      accumulators every weight row and moved them back
+                      (about 30 accumulator moves a row)
   (no fast-math, denormals kept, correctly rounded f32 divide and sqrt: the
   HIP defaults)
 """
@@ -28,6 +33,7 @@ HIPCC_FLAGS = [
     "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall",
     "-Wno-unused-function",
 ]

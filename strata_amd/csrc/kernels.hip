@@ -912,13 +912,30 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         constexpr int kStage = (LH * LW + kTileThreads - 1) / kTileThreads;
         float sv[kStage];
         const int64_t bx = tx0 - R, by = ty0 - R;
-        if (bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1) {
-            const STRATA_GLOBAL float *base = gp(s.p) + (by - s.sy) * s.stride + (bx - s.sx);
+        if (bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1 &&
+            (int64_t)(LH - 1) * s.stride + LW < ((int64_t)1 << 29)) {
+            // Inside the buffer, with the window's byte offsets in 32 bits:
+            // every load at a 32-bit offset from one uniform base, the
+            // offsets stepped from cell to cell (256 cells on: DR rows and
+            // DC columns), so no load waits on address arithmetic. Cells
+            // past the window read its last cell, which is not staged.
+            constexpr int DR = kTileThreads / LW, DC = kTileThreads % LW;
+            const char *base = reinterpret_cast<const char *>(s.p + ((by - s.sy) * s.stride + (bx - s.sx)));
+            const uint32_t st4 = (uint32_t)s.stride * 4u;
+            const uint32_t last = (uint32_t)(LH - 1) * st4 + 4u * (uint32_t)(LW - 1);
+            int row = tid / LW, col = tid - (tid / LW) * LW;
+            uint32_t off = (uint32_t)row * st4 + 4u * (uint32_t)col;
 #pragma unroll
             for (int k = 0; k < kStage; ++k) {
-                const int i = min(tid + k * kTileThreads, LH * LW - 1);
-                const int row = i / LW, col = i - row * LW;
-                sv[k] = base[(int64_t)row * s.stride + col];
+                sv[k] = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (row < LH ? off : last));
+                row += DR;
+                col += DC;
+                off += DR * st4 + 4u * DC;
+                if (col >= LW) {
+                    col -= LW;
+                    row += 1;
+                    off += st4 - 4u * LW;
+                }
             }
         } else {
 #pragma unroll
@@ -1034,29 +1051,34 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         // Column pass for the tile's rows over the staged columns. Unrolled
         // (a compile-time trip count), the LDS reads of several cells are in
         // flight together.
-        auto col_cell = [&](int i) {
+        // The column value of cell i of the tile's rows (every kind but the fit).
+        auto col_value = [&](int i) {
             const int rr = i / LW, cc = i - rr * LW;
+            float a;
             if constexpr (KIND == FK_SEPARABLE) {
                 // ColumnCorrelate with the col taps (focalrow.go:232-242).
-                float a = taps[N] * tile[rr][cc];
+                a = taps[N] * tile[rr][cc];
 #pragma unroll
                 for (int j = 1; j < N; ++j) a = a + taps[N + j] * tile[rr + j][cc];
-                colv[0][rr][cc] = a;
             } else if constexpr (KIND == FK_MEAN) {
-                float a = tile[rr][cc];  // ColumnSum (focalrow.go:244-250)
+                a = tile[rr][cc];  // ColumnSum (focalrow.go:244-250)
 #pragma unroll
                 for (int j = 1; j < N; ++j) a += tile[rr + j][cc];
-                colv[0][rr][cc] = a;
             } else if constexpr (KIND == FK_MIN) {
-                float a = tile[rr][cc];
+                a = tile[rr][cc];
 #pragma unroll
                 for (int j = 1; j < N; ++j) a = go_min(a, tile[rr + j][cc]);
-                colv[0][rr][cc] = a;
-            } else if constexpr (KIND == FK_MAX) {
-                float a = tile[rr][cc];
+            } else {
+                a = tile[rr][cc];
 #pragma unroll
                 for (int j = 1; j < N; ++j) a = go_max(a, tile[rr + j][cc]);
-                colv[0][rr][cc] = a;
+            }
+            return a;
+        };
+        auto col_cell = [&](int i) {
+            const int rr = i / LW, cc = i - rr * LW;
+            if constexpr (KIND != FK_FIT) {
+                colv[0][rr][cc] = col_value(i);
             } else {
                 // Wood fit (terrain/fit.go:137-145): plain, i-weighted and
                 // t2-weighted column sums; ti = j - R, t2 = 3(j-R)^2 - R(R+1).
@@ -1077,7 +1099,26 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
                 }
             }
         };
-        if constexpr (UNROLL) {
+        if constexpr (UNROLL && KIND != FK_FIT) {
+            // Every cell of the thread computed before any is stored: the
+            // stores go to the same LDS array as the reads, so a store
+            // between two cells' reads would keep the later reads waiting.
+            constexpr int kColIters = (kTileY * LW + kTileThreads - 1) / kTileThreads;
+            float cv[kColIters];
+#pragma unroll
+            for (int k = 0; k < kColIters; ++k) {
+                const int i = min(tid + k * kTileThreads, kTileY * LW - 1);
+                cv[k] = col_value(i);
+            }
+#pragma unroll
+            for (int k = 0; k < kColIters; ++k) {
+                const int i = tid + k * kTileThreads;
+                if (i < kTileY * LW) {
+                    const int rr = i / LW, cc = i - rr * LW;
+                    colv[0][rr][cc] = cv[k];
+                }
+            }
+        } else if constexpr (UNROLL) {
             constexpr int kColIters = (kTileY * LW + kTileThreads - 1) / kTileThreads;
 #pragma unroll
             for (int k = 0; k < kColIters; ++k) {
