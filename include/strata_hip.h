@@ -409,8 +409,8 @@ typedef struct strata_tuning {
     int32_t fold_stripe_mb;
     /* Cells per unit, in 1024s, of a call that can be cancelled: the work
      * claimed between two polls of the cancel flag (four units are in
-     * flight). 0: the library's choice, a sixteenth of the raster and at
-     * least 4 Mi cells. 0..2^20. */
+     * flight; a call of one unit runs as one launch). 0: the library's
+     * choice, a sixteenth of the raster and at least 1 Gi cells. 0..2^20. */
     int32_t unit_kcells;
     /* For tests of cancellation: when > 0, a strata_process call with a
      * cancel flag claims at most this many units and then ends as if its

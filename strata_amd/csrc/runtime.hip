@@ -568,8 +568,11 @@ int run_rect(const Job &J, const Operands &op, const Rect &rc, hipStream_t st)
 bool cancelled(const strata_options *o) { return o && o->cancel && *o->cancel != 0; }
 
 // Device work to claim between cancellation checks: whole bands of the
-// plan, within one tile, at least this many cells.
-constexpr int64_t kUnitCells = 1 << 22;
+// plan, within one tile, at least this many cells (about 1.6 ms of a
+// radius-1 sweep). Each unit is a launch of its own, which drains the
+// device at its end (about 10 us), so units stay large: a call of one unit
+// runs as the plain call does.
+constexpr int64_t kUnitCells = (int64_t)1 << 30;
 
 int check_operands(const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
                    const strata_options *opts)
@@ -1326,44 +1329,6 @@ int device_to_host_2d(void *host, size_t hpitch, const void *dev, size_t dpitch,
     return STRATA_OK;
 }
 
-// Second streams of cancellable calls, kept per device for reuse (default
-// priority, non-blocking). A call takes one for its duration.
-class SideStreams {
-  public:
-    static SideStreams &get()
-    {
-        static SideStreams *s = new SideStreams();
-        return *s;
-    }
-    int acquire(hipStream_t *out)
-    {
-        int dev = 0;
-        HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            auto it = free_.find(dev);
-            if (it != free_.end()) {
-                *out = it->second;
-                free_.erase(it);
-                return STRATA_OK;
-            }
-        }
-        HIP_TRY(hipStreamCreateWithFlags(out, hipStreamNonBlocking), "creating a stream");
-        return STRATA_OK;
-    }
-    void release(hipStream_t s)
-    {
-        int dev = 0;
-        if (!s || hipGetDevice(&dev) != hipSuccess) return;
-        std::lock_guard<std::mutex> lk(mu_);
-        free_.emplace(dev, s);
-    }
-
-  private:
-    std::mutex mu_;
-    std::multimap<int, hipStream_t> free_;
-};
-
 // Device memory for a call's buffers from the library's stream-ordered pool.
 int pool_alloc(void **p, size_t bytes, hipStream_t st, const char *what)
 {
@@ -1383,81 +1348,80 @@ int64_t unit_target(int64_t W, int64_t H)
     return std::max<int64_t>(kUnitCells, cdiv(W * H, 16));
 }
 
-// A cancellable call: units of the plan (plan_unit) claimed in order,
-// alternately on the call's stream and on a second stream, kUnitsAhead in
-// flight. Before claiming unit k the call waits for unit k-kUnitsAhead, so
-// while the host waits, the device holds the next units on both streams:
-// one unit's last waves overlap the next unit's first ones and the device
-// never drains between units. The flag is checked before every claim and a
-// claimed unit always finishes, so the finished bands are a prefix of the
-// plan (internal/exec/worker.go:144-181, internal/exec/doc.go:165-183).
-// Units write disjoint rows; mask words shared at a unit boundary take
-// atomic updates of each unit's own bits. Adds the finished units' band
-// Stats to s; *any tells whether a unit ran.
+// A cancellable call: units of the plan (plan_unit) claimed in order on the
+// call's stream, kUnitsAhead in flight. Before claiming unit k the call
+// waits for unit k-kUnitsAhead, so while the host waits the device holds
+// the next units. The flag is checked before every claim and a claimed
+// unit always finishes, so the finished bands are a prefix of the plan
+// (internal/exec/worker.go:144-181, internal/exec/doc.go:165-183). A plan
+// of one unit is one launch. Everything runs on the call's stream: a
+// second stream overlapped the ends of small units, but waits between two
+// hardware queues cost about 50 us a call, and the runtime maps streams
+// onto few hardware queues, so another stream of the caller's could queue
+// behind the library's. Adds the finished units' band Stats to s; *any
+// tells whether a unit ran.
 constexpr int kUnitsAhead = 4;
 int run_units(const Job &J, const Operands &op, const Plan &p, const strata_options *opts, hipStream_t st,
               int nsrc, strata_stats &s, bool *any)
 {
     const int64_t W = op.W, H = op.H;
     const int64_t target = unit_target(W, H);
-    hipEvent_t ev[kUnitsAhead + 1] = {};  // one per unit in flight, then the join
-    hipStream_t lane[2] = {st, nullptr};
-    // On every return the call's stream waits for the second stream's work,
-    // so the call's buffers (freed in order on its stream) outlive it.
-    struct Guard {
-        hipEvent_t *e;
-        hipStream_t *lane;
-        ~Guard()
-        {
-            hipEvent_t join = e[kUnitsAhead];
-            if (lane[1]) {
-                if (join && hipEventRecord(join, lane[1]) == hipSuccess) (void)hipStreamWaitEvent(lane[0], join, 0);
-                SideStreams::get().release(lane[1]);
-            }
-            for (int k = 0; k <= kUnitsAhead; ++k)
-                if (e[k]) (void)hipEventDestroy(e[k]);
-        }
-    } guard{ev, lane};
-    for (int k = 0; k <= kUnitsAhead; ++k)
-        HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "unit event");
-    int rc = SideStreams::get().acquire(&lane[1]);
-    if (rc) return rc;
-    // The second stream starts after everything the call's stream holds
-    // (staged operands included).
-    HIP_TRY(hipEventRecord(ev[kUnitsAhead], st), "unit event");
-    HIP_TRY(hipStreamWaitEvent(lane[1], ev[kUnitsAhead], 0), "ordering the unit streams");
-    Unit pend[kUnitsAhead];
-    bool live[kUnitsAhead] = {};
-    auto finish = [&](int k) -> int {
-        if (!live[k]) return STRATA_OK;
-        live[k] = false;
-        HIP_TRY(hipEventSynchronize(ev[k]), "waiting for a unit");
-        for (int64_t i = pend[k].b0; i < pend[k].b1; ++i)
+    const int32_t limit = get_tuning().stop_after_units;
+    int rc = STRATA_OK;
+    const Unit first = plan_unit(p, 0, target);
+    if (first.b1 >= p.bands) {
+        if (cancelled(opts)) return STRATA_CANCELLED;
+        if ((rc = run_rect(J, op, first.rc, st))) return rc;
+        *any = true;
+        for (int64_t i = first.b0; i < first.b1; ++i)
             band_stats(s, J.r, J.rmin, W, H, nsrc, op.nout, plan_band(p, i));
         return STRATA_OK;
+    }
+    hipEvent_t ev[kUnitsAhead] = {};  // one per unit in flight
+    struct Guard {
+        hipEvent_t *e;
+        ~Guard()
+        {
+            for (int k = 0; k < kUnitsAhead; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } guard{ev};
+    for (int k = 0; k < kUnitsAhead; ++k)
+        HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "unit event");
+    Unit pend[kUnitsAhead];
+    bool live[kUnitsAhead] = {};
+    auto stats_of = [&](int k) {
+        live[k] = false;
+        for (int64_t i = pend[k].b0; i < pend[k].b1; ++i)
+            band_stats(s, J.r, J.rmin, W, H, nsrc, op.nout, plan_band(p, i));
     };
     int status = STRATA_OK;
     int64_t n = 0;
     for (int64_t i = 0; i < p.bands; ++n) {
         const int k = (int)(n % kUnitsAhead);
-        if ((rc = finish(k))) return rc;
-        const int32_t limit = get_tuning().stop_after_units;
+        if (live[k]) {
+            HIP_TRY(hipEventSynchronize(ev[k]), "waiting for a unit");
+            stats_of(k);
+        }
         if (cancelled(opts) || (limit > 0 && n >= limit)) {
             status = STRATA_CANCELLED;
             break;
         }
         const Unit u = plan_unit(p, i, target);
-        hipStream_t ls = lane[n & 1];
-        if ((rc = run_rect(J, op, u.rc, ls))) return rc;
-        HIP_TRY(hipEventRecord(ev[k], ls), "unit event");
+        if ((rc = run_rect(J, op, u.rc, st))) return rc;
+        HIP_TRY(hipEventRecord(ev[k], st), "unit event");
         pend[k] = u;
         live[k] = true;
         *any = true;
         i = u.b1;
     }
-    // Claimed units always finish, cancelled or not, in claim order.
-    for (int64_t j = 0; j < kUnitsAhead; ++j)
-        if ((rc = finish((int)((n + j) % kUnitsAhead)))) return rc;
+    // Claimed units always finish, cancelled or not: one wait, then their
+    // Stats in claim order.
+    HIP_TRY(hipStreamSynchronize(st), "waiting for the units");
+    for (int64_t j = 0; j < kUnitsAhead; ++j) {
+        const int k = (int)((n + j) % kUnitsAhead);
+        if (live[k]) stats_of(k);
+    }
     return status;
 }
 

@@ -1,8 +1,8 @@
 """strata_process over host operands and with a live context.
 
 - Host, device and mixed operands give the oracle's bits; a cancellable
-  call runs as units of the plan claimed in order (four in flight over two
-  streams), here at units of 1024 cells that split rows between units and
+  call runs as units of the plan claimed in order (four in flight on the
+  call's stream), here at small units that split rows between units and
   share mask words across them.
 - A context done before the call writes nothing, host operands included
   (internal/exec/doc.go:165-178: "A context that is done before the call
@@ -235,9 +235,9 @@ def test_cancel_mid_run_leaves_band_prefix(gpu, plan, where):
 
 @pytest.mark.parametrize("plan", [(0, 0), (256, 256), (1000, 3)])
 def test_cancellable_device_call_in_many_units(gpu, plan):
-    """Device operands with a live context run as units on two streams; at
-    units of 16384 cells a 1531x1207 window is hundreds of launches, each
-    unit's bits the oracle's."""
+    """Device operands with a live context run as units on the call's
+    stream; at units of 16384 cells a 1531x1207 window is hundreds of
+    launches, each unit's bits the oracle's."""
     for name in ("slope", "mean3", "correlate2"):
         k = K[name]
         sc = build_scene("windows", k, 23, 1531, 1207)
