@@ -1052,26 +1052,28 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         // (a compile-time trip count), the LDS reads of several cells are in
         // flight together.
         // The column value of cell i of the tile's rows (every kind but the fit).
+        // Cell i of the tile's rows is tile[i / LW][i % LW], at lds[i]; the
+        // cells below it are LW floats on.
         auto col_value = [&](int i) {
-            const int rr = i / LW, cc = i - rr * LW;
+            const float *t = lds + i;
             float a;
             if constexpr (KIND == FK_SEPARABLE) {
                 // ColumnCorrelate with the col taps (focalrow.go:232-242).
-                a = taps[N] * tile[rr][cc];
+                a = taps[N] * t[0];
 #pragma unroll
-                for (int j = 1; j < N; ++j) a = a + taps[N + j] * tile[rr + j][cc];
+                for (int j = 1; j < N; ++j) a = a + taps[N + j] * t[j * LW];
             } else if constexpr (KIND == FK_MEAN) {
-                a = tile[rr][cc];  // ColumnSum (focalrow.go:244-250)
+                a = t[0];  // ColumnSum (focalrow.go:244-250)
 #pragma unroll
-                for (int j = 1; j < N; ++j) a += tile[rr + j][cc];
+                for (int j = 1; j < N; ++j) a += t[j * LW];
             } else if constexpr (KIND == FK_MIN) {
-                a = tile[rr][cc];
+                a = t[0];
 #pragma unroll
-                for (int j = 1; j < N; ++j) a = go_min(a, tile[rr + j][cc]);
+                for (int j = 1; j < N; ++j) a = go_min(a, t[j * LW]);
             } else {
-                a = tile[rr][cc];
+                a = t[0];
 #pragma unroll
-                for (int j = 1; j < N; ++j) a = go_max(a, tile[rr + j][cc]);
+                for (int j = 1; j < N; ++j) a = go_max(a, t[j * LW]);
             }
             return a;
         };
@@ -1110,13 +1112,11 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
                 const int i = min(tid + k * kTileThreads, kTileY * LW - 1);
                 cv[k] = col_value(i);
             }
+            float *cw = lds + LH * LW;  // colv[0][i / LW][i % LW]
 #pragma unroll
             for (int k = 0; k < kColIters; ++k) {
                 const int i = tid + k * kTileThreads;
-                if (i < kTileY * LW) {
-                    const int rr = i / LW, cc = i - rr * LW;
-                    colv[0][rr][cc] = cv[k];
-                }
+                if (i < kTileY * LW) cw[i] = cv[k];
             }
         } else if constexpr (UNROLL) {
             constexpr int kColIters = (kTileY * LW + kTileThreads - 1) / kTileThreads;
@@ -1200,6 +1200,25 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
         }
     }
 
+    if constexpr (UNROLL && KIND != FK_FIT) {
+        // A tile inside the region and the interior stores every value,
+        // at 32-bit offsets from the wave's first row, with no per-cell
+        // tests.
+        const bool whole = tx0 + kTileX <= rg.ox + rg.w && ty0 + kTileY <= rg.oy + rg.h && tx0 >= R &&
+                           tx0 + kTileX <= rg.W - R && ty0 >= R && ty0 + kTileY <= rg.H - R &&
+                           (int64_t)kTileY * d0.stride < ((int64_t)1 << 29);
+        if (whole) {
+            char *ob = reinterpret_cast<char *>(d0.p + ((ty0 + rbase - rg.oy) * d0.stride + (tx0 - rg.ox)));
+            const uint32_t st4 = (uint32_t)d0.stride * 4u;
+            uint32_t off = 4u * (uint32_t)lane;
+#pragma unroll
+            for (int o = 0; o < kRowsPerWave; ++o) {
+                *reinterpret_cast<STRATA_GLOBAL float *>(gp(ob) + off) = out0[o];
+                off += st4;
+            }
+            return;
+        }
+    }
     // Store: kernel values over interior cells, NaN over the edge ring.
     const bool col_in = x >= rg.ox && x < rg.ox + rg.w;
     const bool col_interior = x >= R && x < rg.W - R;
