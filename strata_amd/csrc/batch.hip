@@ -209,7 +209,8 @@ ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, con
         BatchTask t = tasks[i];
         batch_units(ops[t.op], t, rows);
         const int64_t data = (int64_t)t.blocks_x * cdiv64(t.rg.h, t.rows);
-        const int64_t vrows = t.vgroup ? kVgRows * (64 / (int64_t)t.vgroup) : kVsRows;  // rows per validity unit
+        const int64_t vrows =
+            (t.vgroup ? kVgRows * (64 / (int64_t)t.vgroup) : kVsRows) * kVUnitChunks;  // rows per validity unit
         t.vunits = masked[i] ? (int32_t)((int64_t)t.vwaves_x * cdiv64(t.rg.h, vrows)) : 0;
         const int f = ops[t.op].kind == BK_HORN ? BF_HORN : BF_FOCAL;
         t.u0 = u[f];

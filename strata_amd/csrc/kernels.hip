@@ -1785,6 +1785,11 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
     int64_t ob = dst.off + (y0 - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, y0)
     const bool whole = wx > 0 && ((rg.w - 1) >> 5) >= wx * kVsOut + kVsOut;
     const bool store_lane = lane >= 2 && lane < 2 + kVsOut;
+    // Rows that follow each other in the output's bits (stride = width), all
+    // of each row in this wave: a row's last word is the next row's first,
+    // so its bits are carried (cv, cown) into that word, stored whole once.
+    const bool chain = dst.stride == rg.w && wx == 0 && ((rg.w + 30) >> 5) + 2 < 2 + kVsOut;
+    uint32_t cv = 0u, cown = 0u;
 #pragma unroll
     for (int i = 0; i < kVsRows; ++i) {
         const int64_t y = y0 + i;
@@ -1819,17 +1824,25 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
             const int64_t oe = ob + rg.w;  // bit after the row
             const int32_t last = clamp32(((oe - 1) >> 5) - wb, -1, 64);  // lane of the row's last word
             const uint32_t e = (uint32_t)(oe & 31);
-            if (store_lane && lane <= last) {
-                uint32_t own = ~0u;
-                if (wx == 0 && lane == 2) own &= ~0u << s;             // the row's first word
-                if (e != 0 && lane == last) own &= (1u << e) - 1u;     // the row's last word
+            uint32_t own = ~0u, val = out;
+            if (wx == 0 && lane == 2) own &= ~0u << s;             // the row's first word
+            if (e != 0 && lane == last) own &= (1u << e) - 1u;     // the row's last word
+            if (lane == 2 && cown != 0u) {                         // the previous row's last bits
+                val = (out & own) | cv;
+                own |= cown;
+            }
+            // The row's last word waits for the next row's first bits.
+            const bool carry = chain && e != 0 && i + 1 < kVsRows && y + 1 < y1;
+            if (store_lane && lane <= last && !(carry && lane == last)) {
                 if (own == ~0u) {
-                    rowd[lane] = out;
+                    rowd[lane] = val;
                 } else {
-                    atomicAnd(rowd_flat + lane, ~own | (out & own));
-                    atomicOr(rowd_flat + lane, out & own);
+                    atomicAnd(rowd_flat + lane, ~own | (val & own));
+                    atomicOr(rowd_flat + lane, val & own);
                 }
             }
+            cv = carry ? (uint32_t)__builtin_amdgcn_readlane((int)(val & own), last) : 0u;
+            cown = carry ? (uint32_t)__builtin_amdgcn_readlane((int)own, last) : 0u;
         }
         ob += dst.stride;
     }
@@ -1940,6 +1953,12 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
     }
 
     int64_t ob = dst.off + (ys - dst.y0) * dst.stride + (rg.ox - dst.x0);  // bit of cell (ox, ys)
+    // Rows that follow each other in the output's bits (stride = width): a
+    // row's last word is the next row's first, so the group carries its
+    // bits (cv, cown, at the group's lane 2) into that word, stored whole
+    // once.
+    const bool chain = dst.stride == rg.w;
+    uint32_t cv = 0u, cown = 0u;
 #pragma unroll
     for (int i = 0; i < kVgRows; ++i) {
         const int64_t y = ys + i;
@@ -1968,18 +1987,31 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
         const int64_t oe = ob + rg.w;  // bit after the row
         const int32_t last = clamp32(((oe - 1) >> 5) - wb, -1, 64);  // lane of the row's last word
         const uint32_t e = (uint32_t)(oe & 31);
-        if (y < ye && l >= 2 && l <= last) {
-            uint32_t own = ~0u;
-            if (l == 2) own &= ~0u << s;                  // the row's first word
-            if (e != 0 && l == last) own &= (1u << e) - 1u;  // the row's last word
+        uint32_t own = ~0u, val = out;
+        if (l == 2) own &= ~0u << s;                     // the row's first word
+        if (e != 0 && l == last) own &= (1u << e) - 1u;  // the row's last word
+        if (l == 2 && cown != 0u) {                      // the previous row's last bits
+            val = (out & own) | cv;
+            own |= cown;
+        }
+        // The row's last word waits for the next row's first bits.
+        const bool carry = chain && e != 0 && i + 1 < kVgRows && y + 1 < ye;
+        if (y < ye && l >= 2 && l <= last && !(carry && l == last)) {
             uint32_t *word = dbits + wb + l;
             if (own == ~0u) {
-                *gp(word) = out;
+                *gp(word) = val;
             } else {
-                atomicAnd(word, ~own | (out & own));
-                atomicOr(word, out & own);
+                atomicAnd(word, ~own | (val & own));
+                atomicOr(word, val & own);
             }
         }
+        // Lane 2 of each group takes its group's last word (group lanes stay
+        // inside the wave: g * G + last < 64 whenever a row is carried).
+        const int from = 4 * min(max(lane - l + last, 0), 63);
+        const uint32_t tv = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)(val & own));
+        const uint32_t to = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)own);
+        cv = carry ? tv : 0u;
+        cown = carry ? to : 0u;
         ob += dst.stride;
     }
 }
@@ -2108,24 +2140,33 @@ __device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &
 // One validity wave of a task per unit.
 __device__ __forceinline__ void batch_validity(const BatchOp &op, const BatchTask &t, int64_t w)
 {
+    const int64_t end = t.rg.oy + t.rg.h;
     if (t.vgroup) {
-        // 64 / vgroup row blocks of kVgRows rows per unit.
+        // kVUnitChunks chunks of 64 / vgroup row blocks of kVgRows rows.
         const int G = (int)t.vgroup;
-        const int64_t y0 = t.rg.oy + w * (int64_t)(kVgRows * (64 / G)), y1 = t.rg.oy + t.rg.h;
-        switch (op.radius) {
-        case 1: validity_groups<1>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
-        case 2: validity_groups<2>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
-        default: validity_groups<3>(t.rg, t.sm, t.has_src, t.dm, y0, y1, G); break;
+        const int64_t chunk = kVgRows * (64 / G);
+        for (int c = 0; c < kVUnitChunks; ++c) {
+            const int64_t y0 = t.rg.oy + (w * kVUnitChunks + c) * chunk;
+            if (y0 >= end) break;
+            switch (op.radius) {
+            case 1: validity_groups<1>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
+            case 2: validity_groups<2>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
+            default: validity_groups<3>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
+            }
         }
         return;
     }
-    const int64_t wx = w % t.vwaves_x, strip = w / t.vwaves_x;
-    const int64_t y0 = t.rg.oy + strip * kVsRows;
-    const int64_t y1 = min(y0 + (int64_t)kVsRows, t.rg.oy + t.rg.h);
-    switch (op.radius) {
-    case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-    case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-    default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+    // kVUnitChunks strips of kVsRows rows of the unit's word column wx.
+    const int64_t wx = w % t.vwaves_x, strip = (w / t.vwaves_x) * kVUnitChunks;
+    for (int c = 0; c < kVUnitChunks; ++c) {
+        const int64_t y0 = t.rg.oy + (strip + c) * kVsRows;
+        if (y0 >= end) break;
+        const int64_t y1 = min(y0 + (int64_t)kVsRows, end);
+        switch (op.radius) {
+        case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+        case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+        default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+        }
     }
 }
 

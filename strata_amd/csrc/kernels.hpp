@@ -242,6 +242,11 @@ constexpr int kVsRows = 8;
 // Rows per row group of a narrow region's validity units (eight: sixteen
 // raised the wave kernel's registers).
 constexpr int kVgRows = 8;
+// A batch's validity unit: this many consecutive row chunks (of kVsRows
+// rows, or of a narrow region's row groups) of one task, run one after the
+// other by the wave that claims the unit, so the claim, the workgroup's
+// barrier and the task's fields are paid once for all of them.
+constexpr int kVUnitChunks = 4;
 
 struct BatchTask {
     Region rg;
