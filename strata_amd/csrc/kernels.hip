@@ -1719,12 +1719,16 @@ __device__ __forceinline__ int32_t clamp32(int64_t v, int32_t lo, int32_t hi)
 // [wx*kVsOut, wx*kVsOut + kVsOut) of each of the region's rows. wx, y0 and
 // y1 are wave-uniform: every row and word offset below is scalar, advanced
 // by one stride per row, and lanes only add their own 32-bit word index.
+// cv, cown carry the bits of a row's last word into the next row's first
+// (see chain below), also into the wave's next call when it continues with
+// the rows from y1 up to yend.
 // Waves whose words lie inside the source rows and the output rows (all but
 // the first and last wave of a row, and strips at the buffer's top and
 // bottom) take unchecked loads and whole-word stores.
 template <int R>
 __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView &src, int has_src,
-                                               const MaskView &dst, int64_t wx, int64_t y0, int64_t y1)
+                                               const MaskView &dst, int64_t wx, int64_t y0, int64_t y1,
+                                               int64_t yend, uint32_t &cv, uint32_t &cown)
 {
     constexpr int NL = kVsRows + 2 * R;  // source rows y0-R .. y0+kVsRows-1+R
     const int lane = (int)(threadIdx.x & 63);
@@ -1789,7 +1793,6 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
     // of each row in this wave: a row's last word is the next row's first,
     // so its bits are carried (cv, cown) into that word, stored whole once.
     const bool chain = dst.stride == rg.w && wx == 0 && ((rg.w + 30) >> 5) + 2 < 2 + kVsOut;
-    uint32_t cv = 0u, cown = 0u;
 #pragma unroll
     for (int i = 0; i < kVsRows; ++i) {
         const int64_t y = y0 + i;
@@ -1832,7 +1835,7 @@ __device__ __forceinline__ void validity_strip(const Region &rg, const MaskView 
                 own |= cown;
             }
             // The row's last word waits for the next row's first bits.
-            const bool carry = chain && e != 0 && i + 1 < kVsRows && y + 1 < y1;
+            const bool carry = chain && e != 0 && y + 1 < yend;
             if (store_lane && lane <= last && !(carry && lane == last)) {
                 if (own == ~0u) {
                     rowd[lane] = val;
@@ -1859,7 +1862,8 @@ __global__ __launch_bounds__(256) void validity_kernel(Region rg, MaskView src, 
     const int64_t y0 = rg.oy + (int64_t)strip * kVsRows;
     const int64_t y1 = min(y0 + (int64_t)kVsRows, rg.oy + rg.h);
     if (y0 >= y1) return;  // past the last strip
-    validity_strip<R>(rg, src, has_src, dst, wx, y0, y1);
+    uint32_t cv = 0u, cown = 0u;
+    validity_strip<R>(rg, src, has_src, dst, wx, y0, y1, y1, cv, cown);
 }
 
 // A radius-1 Horn op with one masked output, data and validity in one
@@ -1878,7 +1882,8 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_valid(HornArgs a, Re
         const int64_t y0 = rg.oy + (int64_t)strip * kVsRows;
         const int64_t y1 = min(y0 + (int64_t)kVsRows, rg.oy + rg.h);
         if (y0 >= y1) return;  // past the last strip
-        validity_strip<1>(rg, vsrc, has_src, vdst, wx, y0, y1);
+        uint32_t cv = 0u, cown = 0u;
+        validity_strip<1>(rg, vsrc, has_src, vdst, wx, y0, y1, y1, cv, cown);
         return;
     }
     const uint32_t t = xcd_tile(b, data_blocks);
@@ -1901,8 +1906,9 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_valid(HornArgs a, Re
 
 // validity_strip's rule for a narrow region, whose row (its output words
 // and four helper lanes) fits in G <= 32 lanes: the wave's lanes form
-// 64 / G groups of G lanes, group g taking the region's rows
-// [y0 + kVgRows g, y0 + kVgRows (g + 1)) below y1, its lane l standing for
+// 64 / G groups of G lanes, group g running through the region's rows
+// [y0 + run g, y0 + run (g + 1)) below y1 in chunks of kVgRows, this call
+// taking chunk c, its lane l standing for
 // validity_strip's lane l of the region's first word group (cell word
 // xc + 32 l with xc = rg.ox - 64). Lanes past the last group only compute.
 // Row and word offsets differ between groups, so they are per lane; every
@@ -1911,13 +1917,15 @@ __global__ __launch_bounds__(kSweepThreads) void horn_sweep_valid(HornArgs a, Re
 // atomics (a narrow output's rows share words).
 template <int R>
 __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView &src, int has_src,
-                                                const MaskView &dst, int64_t y0, int64_t y1, int G)
+                                                const MaskView &dst, int64_t y0, int64_t y1, int G, int64_t run,
+                                                int c, uint32_t &cv, uint32_t &cown)
 {
     constexpr int NL = kVgRows + 2 * R;
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / G, l = lane - g * G;
-    const int64_t ys = y0 + (int64_t)g * kVgRows;                  // the group's first row
-    const int64_t ye = g < 64 / G ? min(ys + kVgRows, y1) : ys;     // and its end
+    const int64_t ys = y0 + (int64_t)g * run + (int64_t)c * kVgRows;  // the chunk's first row
+    const int64_t rend = min(y0 + (int64_t)(g + 1) * run, y1);       // the group's run ends here
+    const int64_t ye = g < 64 / G ? max(ys, min(ys + kVgRows, rend)) : ys;
     const int64_t xc = rg.ox - 64;
     const uint32_t *sbits = reinterpret_cast<const uint32_t *>(src.bits);
     uint32_t *dbits = reinterpret_cast<uint32_t *>(dst.bits);
@@ -1956,9 +1964,8 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
     // Rows that follow each other in the output's bits (stride = width): a
     // row's last word is the next row's first, so the group carries its
     // bits (cv, cown, at the group's lane 2) into that word, stored whole
-    // once.
+    // once, through the group's run (into its next chunk's call).
     const bool chain = dst.stride == rg.w;
-    uint32_t cv = 0u, cown = 0u;
 #pragma unroll
     for (int i = 0; i < kVgRows; ++i) {
         const int64_t y = ys + i;
@@ -1995,7 +2002,7 @@ __device__ __forceinline__ void validity_groups(const Region &rg, const MaskView
             own |= cown;
         }
         // The row's last word waits for the next row's first bits.
-        const bool carry = chain && e != 0 && i + 1 < kVgRows && y + 1 < ye;
+        const bool carry = chain && e != 0 && y + 1 < rend;
         if (y < ye && l >= 2 && l <= last && !(carry && l == last)) {
             uint32_t *word = dbits + wb + l;
             if (own == ~0u) {
@@ -2141,31 +2148,34 @@ __device__ __forceinline__ void batch_focal(const BatchOp &op, const BatchTask &
 __device__ __forceinline__ void batch_validity(const BatchOp &op, const BatchTask &t, int64_t w)
 {
     const int64_t end = t.rg.oy + t.rg.h;
+    uint32_t cv = 0u, cown = 0u;  // a row's last bits, carried into the next row's first word
     if (t.vgroup) {
-        // kVUnitChunks chunks of 64 / vgroup row blocks of kVgRows rows.
+        // 64 / vgroup groups, each through a run of kVUnitChunks chunks of
+        // kVgRows rows, one chunk per call.
         const int G = (int)t.vgroup;
-        const int64_t chunk = kVgRows * (64 / G);
+        const int64_t run = (int64_t)kVgRows * kVUnitChunks;
+        const int64_t y0 = t.rg.oy + w * (64 / G) * run;
         for (int c = 0; c < kVUnitChunks; ++c) {
-            const int64_t y0 = t.rg.oy + (w * kVUnitChunks + c) * chunk;
-            if (y0 >= end) break;
+            if (y0 + (int64_t)c * kVgRows >= end) break;
             switch (op.radius) {
-            case 1: validity_groups<1>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
-            case 2: validity_groups<2>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
-            default: validity_groups<3>(t.rg, t.sm, t.has_src, t.dm, y0, end, G); break;
+            case 1: validity_groups<1>(t.rg, t.sm, t.has_src, t.dm, y0, end, G, run, c, cv, cown); break;
+            case 2: validity_groups<2>(t.rg, t.sm, t.has_src, t.dm, y0, end, G, run, c, cv, cown); break;
+            default: validity_groups<3>(t.rg, t.sm, t.has_src, t.dm, y0, end, G, run, c, cv, cown); break;
             }
         }
         return;
     }
     // kVUnitChunks strips of kVsRows rows of the unit's word column wx.
     const int64_t wx = w % t.vwaves_x, strip = (w / t.vwaves_x) * kVUnitChunks;
+    const int64_t yend = min(t.rg.oy + (strip + kVUnitChunks) * kVsRows, end);
     for (int c = 0; c < kVUnitChunks; ++c) {
         const int64_t y0 = t.rg.oy + (strip + c) * kVsRows;
         if (y0 >= end) break;
         const int64_t y1 = min(y0 + (int64_t)kVsRows, end);
         switch (op.radius) {
-        case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-        case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
-        default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1); break;
+        case 1: validity_strip<1>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1, yend, cv, cown); break;
+        case 2: validity_strip<2>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1, yend, cv, cown); break;
+        default: validity_strip<3>(t.rg, t.sm, t.has_src, t.dm, wx, y0, y1, yend, cv, cown); break;
         }
     }
 }
