@@ -345,8 +345,9 @@ def paths_bench(args, dem, out, src, sink, slope, N, dev, log):
       exec.Pipeline call, SlopeChunked-style over the resident DEM, against
       the three calls it replaces, with the outputs compared bit for bit;
     - host memory: the DEM and the output in host (pageable) memory, as a
-      Go caller hands them over, through ProcessN (staged whole through
-      pinned pieces) and through ProcessChunked with TileHeight 256
+      Go caller hands them over, through ProcessN (the library streams the
+      rasters through its tile pipeline) and through ProcessChunked with
+      TileHeight 256
       (MemorySource/MemorySink windows copied by the library into pinned
       tile buffers), in Gcells/s and in GB/s over PCIe; the output is
       compared with the device-resident result bit for bit."""
@@ -449,7 +450,8 @@ def paths_bench(args, dem, out, src, sink, slope, N, dev, log):
                     "pcie_gb_s": round(chunk_b / h_chunk / 1e9, 2), "identical": ok_chunk,
                     "tile_height": args.tile_height},
         "what": f"terrain.Slope {N}x{N} with the DEM and the output in host (pageable) memory: ProcessN "
-                f"(whole rasters staged through HBM in pinned pieces) and ProcessChunked (MemorySource/MemorySink "
+                f"(streamed by the library through its tile pipeline, tiles of about 16 MiB) and ProcessChunked "
+                f"(MemorySource/MemorySink "
                 f"windows copied by the library into pinned tile buffers)"}
     log(f"bench: host memory {res['host_memory']}")
     return res

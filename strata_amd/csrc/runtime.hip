@@ -1425,6 +1425,13 @@ int run_units(const Job &J, const Operands &op, const Plan &p, const strata_opti
     return status;
 }
 
+int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata_source *src, int nsrc,
+                    const strata_options *opts, strata_stats *stats);
+
+// Rows of a tile when a plain call's host operands stream through the tile
+// pipeline: about 16 MiB of cells per tile, at least 64 rows.
+int64_t host_tile_rows(int64_t W) { return std::max<int64_t>(64, ((int64_t)16 << 20) / (4 * W)); }
+
 int process(const Job &J, const strata_raster *dst, int ndst, const strata_raster *src, int nsrc,
             const strata_options *opts, strata_stats *stats)
 {
@@ -1442,11 +1449,46 @@ int process(const Job &J, const strata_raster *dst, int ndst, const strata_raste
         return STRATA_CANCELLED;
     }
 
-    // Host operands are mirrored in device memory for the call. A host
-    // output's mirror starts as the caller's raster when the call can be
-    // cancelled, so the bands it never claims go back unchanged; otherwise
-    // every cell is computed and the mirror needs no copy in.
-    bool any_host = src[0].mem == STRATA_MEM_HOST;
+    // Host operands and nothing to poll: the call streams through the
+    // chunked path's tile pipeline (a tile's rows copied in while the device
+    // computes the previous tile and the one before is copied out), which
+    // computes the same cells; its Stats are the plain plan's.
+    bool any_host = false;
+    for (int i = 0; i < nsrc; ++i) any_host = any_host || src[i].mem == STRATA_MEM_HOST;
+    for (int o = 0; o < ndst; ++o) any_host = any_host || dst[o].mem == STRATA_MEM_HOST;
+    if (any_host && !can_cancel && nsrc == 1) {
+        strata_source cs[1] = {};  // one input, as every job here has
+        strata_sink cd[kMaxDst] = {};
+        for (int i = 0; i < nsrc; ++i) {
+            cs[i].width = W;
+            cs[i].height = H;
+            cs[i].masked = src[i].valid != nullptr;
+            cs[i].resident = &src[i];
+        }
+        for (int o = 0; o < ndst; ++o) {
+            cd[o].width = W;
+            cd[o].height = H;
+            cd[o].masked = dst[o].valid != nullptr;
+            cd[o].resident = &dst[o];
+        }
+        strata_options co{};
+        if (opts) co = *opts;
+        co.tile_width = 0;
+        co.tile_height = host_tile_rows(W);
+        co.cancel = nullptr;
+        co.async = 0;
+        strata_stats ignored{};
+        if ((rc = process_chunked(J, cd, ndst, cs, nsrc, &co, &ignored))) return rc;
+        strata_stats s{};
+        for (int64_t b = 0; b < p.bands; ++b) band_stats(s, J.r, J.rmin, W, H, nsrc, ndst, plan_band(p, b));
+        add_stats(stats, s, nsrc, ndst);
+        g_err.clear();
+        return STRATA_OK;
+    }
+
+    // With a cancel flag, host operands are mirrored in device memory for
+    // the call: a host output's mirror starts as the caller's raster, so the
+    // bands the call never claims go back unchanged.
     Staged ssrc, sdst[kMaxDst];
     const strata_raster *dsrc = &src[0];
     Operands op{dsrc, 0, 0, {}, {}, {}, ndst, W, H};
