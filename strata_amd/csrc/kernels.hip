@@ -1410,8 +1410,10 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     constexpr int NP = LH * PPR;
     constexpr int KST = (NP + 255) / 256;
     static_assert(2 * PPR <= PITCH, "staged pairs fit the LDS row");
-    __shared__ __attribute__((aligned(16))) __bf16 hi[LH * PITCH];
-    __shared__ __attribute__((aligned(16))) __bf16 lo[LH * PITCH];
+    // The staged tile's high and low planes; after the products the same
+    // LDS holds a whole tile's outputs on their way to the stores.
+    __shared__ __attribute__((aligned(16))) __bf16 planes[2 * LH * PITCH];
+    __bf16 *hi = planes, *lo = planes + LH * PITCH;
     __shared__ float wts[N * N];
     __shared__ int nonfinite;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1439,6 +1441,9 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
         if (inside) {
             const char *base = reinterpret_cast<const char *>(s.p + ((by - s.sy) * s.stride + (bx - s.sx)));
             const uint32_t st4 = (uint32_t)s.stride * 4u;
+            // A pair is one 8-byte load when every pair starts 8-byte
+            // aligned (LW is even, so a pair is wholly inside or outside).
+            const bool pair8 = (reinterpret_cast<uintptr_t>(base) & 7u) == 0 && (st4 & 7u) == 0;
             uint32_t off = (uint32_t)row * st4 + 8u * (uint32_t)pc;
 #pragma unroll
             for (int k = 0; k < KST; ++k) {
@@ -1446,8 +1451,16 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
                 // Past the last row or column the load is clamped to the
                 // window's first cell and its value replaced by 0.
                 const bool ok0 = row < LH && col < LW, ok1 = row < LH && col + 1 < LW;
-                const float a = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok0 ? off : 0u));
-                const float b = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok1 ? off + 4u : 0u));
+                float a, b;
+                if (pair8) {
+                    typedef float f32x2 __attribute__((ext_vector_type(2)));
+                    const f32x2 ab = *reinterpret_cast<const STRATA_GLOBAL f32x2 *>(gp(base) + (ok0 ? off : 0u));
+                    a = ab.x;
+                    b = ab.y;
+                } else {
+                    a = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok0 ? off : 0u));
+                    b = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok1 ? off + 4u : 0u));
+                }
                 va[k] = ok0 ? a : 0.0f;
                 vb[k] = ok1 ? b : 0.0f;
                 pc += 16;
@@ -1542,6 +1555,28 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     const bool whole = !fix && tx0 + kMfmaTile <= rg.ox + rg.w && ty0 + kMfmaTile <= rg.oy + rg.h && tx0 >= R &&
                        tx0 + kMfmaTile <= rg.W - R && ty0 >= R && ty0 + kMfmaTile <= rg.H - R &&
                        (int64_t)kMfmaTile * d0.stride < ((int64_t)1 << 29);
+    if (whole && (reinterpret_cast<uintptr_t>(d0.p) & 15u) == 0 && (d0.stride & 3) == 0) {
+        // Through LDS, so each thread stores 16 consecutive outputs of one
+        // row with four 16-byte stores (a row of 68 floats puts the 64
+        // lanes of each write in 32 distinct banks, two lanes a bank).
+        constexpr int OP = 68;
+        static_assert(kMfmaTile * OP * 4 <= 2 * LH * PITCH * 2, "the outputs fit the staging planes");
+        float *ot = reinterpret_cast<float *>(planes);
+        __syncthreads();  // every wave is done reading the planes
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) ot[(16 * b + 4 * kg + e) * OP + c0 + n] = acc[b][e];
+        __syncthreads();
+        const int r = tid >> 2, cq = 16 * (tid & 3);
+        char *ob = reinterpret_cast<char *>(d0.p + ((ty0 - rg.oy) * d0.stride + (tx0 - rg.ox)));
+        const uint32_t o0 = (uint32_t)r * ((uint32_t)d0.stride * 4u) + 4u * (uint32_t)cq;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<STRATA_GLOBAL f32x4 *>(gp(ob) + o0 + 16u * q) =
+                *reinterpret_cast<const f32x4 *>(ot + r * OP + cq + 4 * q);
+        return;
+    }
     if (whole) {
         char *ob = reinterpret_cast<char *>(d0.p + ((ty0 - rg.oy) * d0.stride + (tx0 - rg.ox)));
         const uint32_t st4 = (uint32_t)d0.stride * 4u;
