@@ -584,15 +584,11 @@ __device__ __forceinline__ void sweep(const HornArgs &a, const Region &rg, const
             if (((reinterpret_cast<uintptr_t>(rp) + sbase) & 4u) == 0u) {
                 __builtin_nontemporal_store(pair_t{v0, v1}, reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so));
             } else {
-                // Pairs (x - 1, x) from the left lane's second value; the
-                // wave's first and last cells go alone.
-                const float pv = from_left(v1, 0.0f);
-                if (lane != 0)
-                    __builtin_nontemporal_store(pair_t{pv, v0},
-                                                reinterpret_cast<STRATA_GLOBAL pair_t *>(rp + so - 4u));
-                else
-                    *reinterpret_cast<STRATA_GLOBAL float *>(rp + so) = v0;
-                if (lane == 63) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so + 4u) = v1;
+                // Odd 4-byte offset: each lane's two cells as two 4-byte
+                // stores (every lane's cells lie inside the region here);
+                // the cache joins the two halves of each line.
+                __builtin_nontemporal_store(v0, reinterpret_cast<STRATA_GLOBAL float *>(rp + so));
+                __builtin_nontemporal_store(v1, reinterpret_cast<STRATA_GLOBAL float *>(rp + so + 4u));
             }
         } else {
             if (in0) *reinterpret_cast<STRATA_GLOBAL float *>(rp + so) = v0;
