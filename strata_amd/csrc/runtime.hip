@@ -455,9 +455,13 @@ hipEvent_t timing_event(hipStream_t st)
     return e;
 }
 
+// The library's device scratch (defined with the scratch pool below).
+int scratch_alloc(void **p, size_t bytes, hipStream_t st);
+void scratch_free(void *p, hipStream_t st);
+
 // A chain's groups over the output rect rc: every group but the last
 // writes its values over rc grown by its `grow` (clipped to the raster)
-// into a buffer from the library's pool, which the next group reads as its
+// into a buffer of the library's scratch, which the next group reads as its
 // input; the last writes the output. Each stage computes its own edge ring
 // (NaN), which the stages after it carry through, so the cells within the
 // chain's reach of the raster's edge hold NaN as the separate calls leave
@@ -470,10 +474,7 @@ int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t
     struct Free {
         void *&p;
         hipStream_t st;
-        ~Free()
-        {
-            if (p) (void)hipFreeAsync(p, st);
-        }
+        ~Free() { scratch_free(p, st); }
     } guard{prev, st};
     for (size_t g = 0; g < J.chain.size(); ++g) {
         const ChainGroup &G = J.chain[g];
@@ -490,7 +491,8 @@ int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t
             out.x1 = std::min<int64_t>(W, rc.x1 + G.grow);
             out.y1 = std::min<int64_t>(H, rc.y1 + G.grow);
             const int64_t bs = (out.x1 - out.x0 + 63) & ~(int64_t)63;
-            HIP_TRY(scratch_alloc_async(&buf, (size_t)((out.y1 - out.y0) * bs) * 4, st), "a chain's buffer");
+            int rc = scratch_alloc(&buf, (size_t)((out.y1 - out.y0) * bs) * 4, st);
+            if (rc) return rc;
             d.p = (float *)buf;
             d.stride = bs;
         }
@@ -502,7 +504,7 @@ int run_chain_data(const Job &J, const Operands &op, const Rect &rc, hipStream_t
             const SrcView v{d.p, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
             HIP_TRY(launch_data(Q, rg, v, &d, 1, st), "launching a chain's point stage");
         }
-        if (prev) HIP_TRY(hipFreeAsync(prev, st), "freeing a chain's buffer");
+        scratch_free(prev, st);
         prev = buf;
         if (!last) s = SrcView{(const float *)buf, d.stride, out.x0, out.y0, out.x0, out.x1, out.y0, out.y1};
     }
@@ -648,10 +650,100 @@ int scratch_pool(hipMemPool_t *out)
     return STRATA_OK;
 }
 
+// Large scratch (64 MiB and up: rows of outputs that are only folded,
+// mirrors of host rasters, a chain's intermediate values) comes from blocks
+// of the library's own per device, allocated once and reused: taken from
+// the stream-ordered pool, each such buffer was mapped again on every call
+// (a fold-only pass over 1.6 GB ran 0.45 ms slower than one writing the
+// caller's raster). A block goes back to the free list when the call's
+// stream reaches the point where the call gives it up (a host function on
+// the stream), so a later call, on any stream, only takes it once every
+// use is done. strata_trim returns the free blocks.
+constexpr size_t kBigScratch = (size_t)64 << 20;
+struct BigBlock {
+    int dev;
+    size_t bytes;
+};
+std::mutex g_big_mu;
+std::multimap<std::pair<int, size_t>, void *> g_big_free;  // (device, bytes) -> block
+std::map<void *, BigBlock> g_big_used;
+
+int scratch_alloc(void **p, size_t bytes, hipStream_t st)
+{
+    *p = nullptr;
+    if (bytes < kBigScratch) {
+        hipMemPool_t pool = nullptr;
+        int rc = scratch_pool(&pool);
+        if (rc) return rc;
+        HIP_TRY(hipMallocFromPoolAsync(p, bytes, pool, st), "scratch");
+        return STRATA_OK;
+    }
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    {
+        // The smallest free block that holds the request and is at most
+        // twice its size.
+        std::lock_guard<std::mutex> lk(g_big_mu);
+        auto it = g_big_free.lower_bound({dev, bytes});
+        if (it != g_big_free.end() && it->first.first == dev && it->first.second <= 2 * bytes) {
+            *p = it->second;
+            g_big_used[*p] = BigBlock{dev, it->first.second};
+            g_big_free.erase(it);
+            return STRATA_OK;
+        }
+    }
+    const size_t rounded = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+    HIP_TRY(hipMalloc(p, rounded), "scratch block");
+    std::lock_guard<std::mutex> lk(g_big_mu);
+    g_big_used[*p] = BigBlock{dev, rounded};
+    return STRATA_OK;
+}
+
+void big_release(void *p)
+{
+    std::lock_guard<std::mutex> lk(g_big_mu);
+    auto it = g_big_used.find(p);
+    if (it == g_big_used.end()) return;
+    g_big_free.emplace(std::make_pair(it->second.dev, it->second.bytes), p);
+    g_big_used.erase(it);
+}
+
+// Gives p back once st reaches this point.
+void scratch_free(void *p, hipStream_t st)
+{
+    if (!p) return;
+    bool big = false;
+    {
+        std::lock_guard<std::mutex> lk(g_big_mu);
+        big = g_big_used.count(p) != 0;
+    }
+    if (!big) {
+        (void)hipFreeAsync(p, st);
+        return;
+    }
+    if (hipLaunchHostFunc(st, [](void *q) { big_release(q); }, p) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        big_release(p);
+    }
+}
+
+// Frees the free blocks (those of finished calls).
+int big_trim()
+{
+    std::vector<void *> blocks;
+    {
+        std::lock_guard<std::mutex> lk(g_big_mu);
+        for (auto &b : g_big_free) blocks.push_back(b.second);
+        g_big_free.clear();
+    }
+    for (void *b : blocks) HIP_TRY(hipFree(b), "freeing a scratch block");
+    return STRATA_OK;
+}
+
 // A host-memory raster mirrored in device memory for a call: the
 // raster's whole span (cells and padding) and the mask words its bits
-// touch, in memory from the library's stream-ordered pool (reused from call
-// to call, given back when the call's stream reaches the end of the call).
+// touch, in the library's scratch (reused from call to call, given back
+// when the call's stream reaches the end of the call).
 struct Staged {
     strata_raster dev{};
     void *dbuf = nullptr;
@@ -660,8 +752,8 @@ struct Staged {
     hipStream_t st = nullptr;
     ~Staged()
     {
-        if (dbuf) (void)hipFreeAsync(dbuf, st);
-        if (dmask) (void)hipFreeAsync(dmask, st);
+        scratch_free(dbuf, st);
+        scratch_free(dmask, st);
     }
 };
 
@@ -673,21 +765,19 @@ int device_to_host_2d(void *host, size_t hpitch, const void *dev, size_t dpitch,
 
 int stage_in(const strata_raster &h, Staged &s, bool copy_data, hipStream_t st)
 {
-    hipMemPool_t pool = nullptr;
-    int rc = scratch_pool(&pool);
-    if (rc) return rc;
+    int rc = STRATA_OK;
     s.st = st;
     s.dev = h;
     s.dev.mem = STRATA_MEM_DEVICE;
     const int64_t n = data_len(h);
-    HIP_TRY(hipMallocFromPoolAsync(&s.dbuf, (size_t)n * sizeof(float), pool, st), "staging data");
+    if ((rc = scratch_alloc(&s.dbuf, (size_t)n * sizeof(float), st))) return rc;
     s.dev.data = (float *)s.dbuf;
     if (copy_data && (rc = host_to_device(s.dbuf, h.data, (size_t)n * sizeof(float), st))) return rc;
     if (h.valid) {
         s.w0 = h.valid_offset >> 6;
         const int64_t w1 = (h.valid_offset + n - 1) >> 6;
         s.nwords = w1 - s.w0 + 1;
-        HIP_TRY(hipMallocFromPoolAsync((void **)&s.dmask, (size_t)s.nwords * 8, pool, st), "staging mask");
+        if ((rc = scratch_alloc((void **)&s.dmask, (size_t)s.nwords * 8, st))) return rc;
         if ((rc = host_to_device(s.dmask, h.valid + s.w0, (size_t)s.nwords * 8, st))) return rc;
         s.dev.valid = s.dmask;
         s.dev.valid_offset = h.valid_offset - 64 * s.w0;
@@ -775,6 +865,8 @@ int strata_trim(void)
         for (hipMemPool_t p : g_pools)
             if (p) HIP_TRY(hipMemPoolTrimTo(p, 0), "trimming the scratch pool");
     }
+    int rc = big_trim();
+    if (rc) return rc;
     pinned_trim();
     g_err.clear();
     return STRATA_OK;
@@ -1807,21 +1899,18 @@ int process_chunked(const Job &J, const strata_sink *dst, int ndst, const strata
                 std::vector<void *> p;
                 ~AsyncBufs()
                 {
-                    for (void *q : p) (void)hipFreeAsync(q, st);
+                    for (void *q : p) scratch_free(q, st);
                 }
             } abufs{st, {}};
-            hipMemPool_t pool = nullptr;
-            if (any_fold_only && (rc = scratch_pool(&pool))) return rc;
             for (int o = 0; o < ndst; ++o) {
                 if (fold_only(o)) {
                     const int64_t cells = step * W;
                     float *data = nullptr;
                     uint64_t *mask = nullptr;
-                    HIP_TRY(hipMallocFromPoolAsync((void **)&data, (size_t)cells * 4, pool, st), "fold buffer");
+                    if ((rc = scratch_alloc((void **)&data, (size_t)cells * 4, st))) return rc;
                     abufs.p.push_back(data);
                     if (dst[o].masked) {
-                        HIP_TRY(hipMallocFromPoolAsync((void **)&mask, (size_t)cdiv(cells, 64) * 8, pool, st),
-                                "fold mask");
+                        if ((rc = scratch_alloc((void **)&mask, (size_t)cdiv(cells, 64) * 8, st))) return rc;
                         abufs.p.push_back(mask);
                     }
                     dv[o].data = data;
