@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
 }
 
 // ---------------------------------------------------------------------------
-// correlate_mfma: focal Correlate/Convolve as split-BF16x3 matrix products
+// correlate_mfma_pipe: focal Correlate/Convolve as split-BF16x3 matrix products
 // ---------------------------------------------------------------------------
 //
 // For a strip of 16 output columns at x0, weight row i contributes
@@ -1278,10 +1278,10 @@ __global__ __launch_bounds__(kTileThreads) void focal_tile(const Params P, Regio
 // error by about 3*2^-18 of |w*x|: this path is approximate, unlike the
 // FP32 path, which follows the reference's operation order exactly.
 //
-// A workgroup computes a 64x64 tile: wave w takes columns 16w..16w+15 and
-// four 16-row blocks. The input tile and halo are staged in LDS as the hi
-// and lo planes (rows padded so each lane's 8 operand values are one
-// aligned 16-byte read). Operand layouts of the 16x16x32 bf16 MFMA: lane l
+// Each 64x64 output tile is computed by four waves: wave w takes columns
+// 16w..16w+15 and four 16-row blocks. The input tile and halo are staged
+// in LDS as the hi and lo planes (rows padded so each lane's 8 operand
+// values are one aligned 16-byte read). Operand layouts of the 16x16x32 bf16 MFMA: lane l
 // holds A[m = l%16][k = 8(l/16) .. +7], B[k = 8(l/16) .. +7][n = l%16], and
 // C[m = 4(l/16) .. +3][n = l%16].
 //
@@ -1340,7 +1340,7 @@ __device__ __forceinline__ float correlate_cell(const float *taps, const SrcView
 // The Toeplitz operands of every weight row, split: entry (i, lane) holds
 // the lane's B operand of T_i (k = 8(lane/16) .. +7, n = lane % 16) as its
 // bf16 high and low parts. Built once per call by N waves; every workgroup
-// of correlate_mfma then reads it (from L2) instead of building it.
+// of correlate_mfma_pipe then reads it (from L2) instead of building it.
 template <int R>
 __global__ __launch_bounds__(64) void mfma_toeplitz(const Params P, bf16x8 *tab)
 {
@@ -1394,11 +1394,12 @@ __device__ __forceinline__ void split_pair_any(float a, float b, uint32_t &hp, u
 // Below the largest bf16 (0x7f7f), so rounding to bf16 stays finite.
 constexpr float kBf16Safe = 3.38e38f;
 
+// Stages one tile's window (the tile at tx0, ty0 and its halo) into the
+// planes hi and lo, by 256 threads (p = 0..255). Returns whether the
+// thread's values held a non-finite one.
 template <int R>
-__global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg, SrcView s, DstView d0,
-                                                      const bf16x8 *__restrict__ tab)
+__device__ __forceinline__ bool mfma_stage(const SrcView &s, int64_t tx0, int64_t ty0, int p, __bf16 *hi, __bf16 *lo)
 {
-    constexpr int N = 2 * R + 1;
     constexpr int LH = kMfmaTile + 2 * R, LW = kMfmaTile + 2 * R, PITCH = mfma_pitch<R>();
     // Staged columns: the A operands read columns 0..79 of a staged row;
     // those at LW and past it meet zeros of T_i and are staged as 0.
@@ -1406,19 +1407,6 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     constexpr int NP = LH * PPR;
     constexpr int KST = (NP + 255) / 256;
     static_assert(2 * PPR <= PITCH, "staged pairs fit the LDS row");
-    // The staged tile's high and low planes; after the products the same
-    // LDS holds a whole tile's outputs on their way to the stores.
-    __shared__ __attribute__((aligned(16))) __bf16 planes[2 * LH * PITCH];
-    __bf16 *hi = planes, *lo = planes + LH * PITCH;
-    __shared__ float wts[N * N];
-    __shared__ int nonfinite;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t gx = (uint32_t)((rg.w + kMfmaTile - 1) / kMfmaTile), t = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t tx0 = rg.ox + (int64_t)(t % gx) * kMfmaTile;
-    const int64_t ty0 = rg.oy + (int64_t)(t / gx) * kMfmaTile;
-    if (tid == 0) nonfinite = 0;
-    for (int i = tid; i < N * N; i += 256) wts[i] = P.taps[i];
-    __syncthreads();
     // Staging in pairs of columns, every load of the thread issued before
     // any split (the trip count is a compile-time constant). A tile whose
     // staged window lies inside the buffer, with row offsets that fit 32
@@ -1427,79 +1415,53 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
     // all finite and inside the bf16 range (a wave-uniform test) takes the
     // short split. Each pair's high and low parts go to LDS as one 32-bit
     // store each.
-    {
-        const int64_t bx = tx0 - R, by = ty0 - R;
-        const bool inside = bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1 &&
-                            (int64_t)(LH - 1) * s.stride + LW < ((int64_t)1 << 29);
-        float va[KST], vb[KST];
-        // The thread's first pair, then 256 pairs on: 6 rows and 16 pairs.
-        int row = tid / PPR, pc = tid - row * PPR;
-        if (inside) {
-            const char *base = reinterpret_cast<const char *>(s.p + ((by - s.sy) * s.stride + (bx - s.sx)));
-            const uint32_t st4 = (uint32_t)s.stride * 4u;
-            // A pair is one 8-byte load when every pair starts 8-byte
-            // aligned (LW is even, so a pair is wholly inside or outside).
-            const bool pair8 = (reinterpret_cast<uintptr_t>(base) & 7u) == 0 && (st4 & 7u) == 0;
-            uint32_t off = (uint32_t)row * st4 + 8u * (uint32_t)pc;
-#pragma unroll
-            for (int k = 0; k < KST; ++k) {
-                const int col = 2 * pc;
-                // Past the last row or column the load is clamped to the
-                // window's first cell and its value replaced by 0.
-                const bool ok0 = row < LH && col < LW, ok1 = row < LH && col + 1 < LW;
-                float a, b;
-                if (pair8) {
-                    typedef float f32x2 __attribute__((ext_vector_type(2)));
-                    const f32x2 ab = *reinterpret_cast<const STRATA_GLOBAL f32x2 *>(gp(base) + (ok0 ? off : 0u));
-                    a = ab.x;
-                    b = ab.y;
-                } else {
-                    a = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok0 ? off : 0u));
-                    b = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok1 ? off + 4u : 0u));
-                }
-                va[k] = ok0 ? a : 0.0f;
-                vb[k] = ok1 ? b : 0.0f;
-                pc += 16;
-                row += 6;
-                off += 6u * st4 + 128u;
-                if (pc >= PPR) {
-                    pc -= PPR;
-                    row += 1;
-                    off += st4 - 8u * PPR;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < KST; ++k) {
-                const int col = 2 * pc;
-                const bool ok = row < LH;
-                va[k] = ok && col < LW ? load_cell(s, bx + col, by + row) : 0.0f;
-                vb[k] = ok && col + 1 < LW ? load_cell(s, bx + col + 1, by + row) : 0.0f;
-                pc += 16;
-                row += 6;
-                if (pc >= PPR) {
-                    pc -= PPR;
-                    row += 1;
-                }
-            }
-        }
-        bool any = false;
-        uint32_t *hi32 = reinterpret_cast<uint32_t *>(hi), *lo32 = reinterpret_cast<uint32_t *>(lo);
-        row = tid / PPR;
-        pc = tid - row * PPR;
+    const int64_t bx = tx0 - R, by = ty0 - R;
+    const bool inside = bx >= s.lx0 && bx + LW <= s.lx1 && by >= s.ly0 && by + LH <= s.ly1 &&
+                        (int64_t)(LH - 1) * s.stride + LW < ((int64_t)1 << 29);
+    float va[KST], vb[KST];
+    // The thread's first pair, then 256 pairs on: 6 rows and 16 pairs.
+    int row = p / PPR, pc = p - row * PPR;
+    if (inside) {
+        const char *base = reinterpret_cast<const char *>(s.p + ((by - s.sy) * s.stride + (bx - s.sx)));
+        const uint32_t st4 = (uint32_t)s.stride * 4u;
+        // A pair is one 8-byte load when every pair starts 8-byte
+        // aligned (LW is even, so a pair is wholly inside or outside).
+        const bool pair8 = (reinterpret_cast<uintptr_t>(base) & 7u) == 0 && (st4 & 7u) == 0;
+        uint32_t off = (uint32_t)row * st4 + 8u * (uint32_t)pc;
 #pragma unroll
         for (int k = 0; k < KST; ++k) {
-            const bool in = row < LH;
-            const bool plain = __builtin_fabsf(va[k]) < kBf16Safe && __builtin_fabsf(vb[k]) < kBf16Safe;
-            uint32_t hp, lp;
-            if (__builtin_amdgcn_ballot_w64(in && !plain) == 0)
-                split_pair_fast(va[k], vb[k], hp, lp);
-            else
-                split_pair_any(va[k], vb[k], hp, lp, any);
-            if (in) {
-                hi32[row * (PITCH / 2) + pc] = hp;
-                lo32[row * (PITCH / 2) + pc] = lp;
+            const int col = 2 * pc;
+            // Past the last row or column the load is clamped to the
+            // window's first cell and its value replaced by 0.
+            const bool ok0 = row < LH && col < LW, ok1 = row < LH && col + 1 < LW;
+            float a, b;
+            if (pair8) {
+                typedef float f32x2 __attribute__((ext_vector_type(2)));
+                const f32x2 ab = *reinterpret_cast<const STRATA_GLOBAL f32x2 *>(gp(base) + (ok0 ? off : 0u));
+                a = ab.x;
+                b = ab.y;
+            } else {
+                a = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok0 ? off : 0u));
+                b = *reinterpret_cast<const STRATA_GLOBAL float *>(gp(base) + (ok1 ? off + 4u : 0u));
             }
+            va[k] = ok0 ? a : 0.0f;
+            vb[k] = ok1 ? b : 0.0f;
+            pc += 16;
+            row += 6;
+            off += 6u * st4 + 128u;
+            if (pc >= PPR) {
+                pc -= PPR;
+                row += 1;
+                off += st4 - 8u * PPR;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KST; ++k) {
+            const int col = 2 * pc;
+            const bool ok = row < LH;
+            va[k] = ok && col < LW ? load_cell(s, bx + col, by + row) : 0.0f;
+            vb[k] = ok && col + 1 < LW ? load_cell(s, bx + col + 1, by + row) : 0.0f;
             pc += 16;
             row += 6;
             if (pc >= PPR) {
@@ -1507,27 +1469,52 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
                 row += 1;
             }
         }
-        if (any) nonfinite = 1;
     }
-    __syncthreads();
+    bool any = false;
+    uint32_t *hi32 = reinterpret_cast<uint32_t *>(hi), *lo32 = reinterpret_cast<uint32_t *>(lo);
+    row = p / PPR;
+    pc = p - row * PPR;
+#pragma unroll
+    for (int k = 0; k < KST; ++k) {
+        const bool in = row < LH;
+        const bool plain = __builtin_fabsf(va[k]) < kBf16Safe && __builtin_fabsf(vb[k]) < kBf16Safe;
+        uint32_t hp, lp;
+        if (__builtin_amdgcn_ballot_w64(in && !plain) == 0)
+            split_pair_fast(va[k], vb[k], hp, lp);
+        else
+            split_pair_any(va[k], vb[k], hp, lp, any);
+        if (in) {
+            hi32[row * (PITCH / 2) + pc] = hp;
+            lo32[row * (PITCH / 2) + pc] = lp;
+        }
+        pc += 16;
+        row += 6;
+        if (pc >= PPR) {
+            pc -= PPR;
+            row += 1;
+        }
+    }
 
+    return any;
+}
+
+// One wave's products of a staged tile: columns 16*wave.. of the tile, its
+// four 16-row blocks into acc.
+template <int R>
+__device__ __forceinline__ void mfma_products(const __bf16 *hi, const __bf16 *lo, const bf16x8 *tab, int lane,
+                                              int wave, f32x4 (&acc)[4])
+{
+    constexpr int N = 2 * R + 1, PITCH = mfma_pitch<R>();
     const int n = lane & 15, kg = lane >> 4;
     const int c0 = wave * 16;
-    f32x4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     // Each block accumulates hi*hi, hi*lo, lo*hi per weight row, in that
-    // order; the four blocks' products interleave. Weight row i + 1's
-    // operands load while row i's products run. Not unrolled: unrolled,
-    // every weight row's operands would be held in registers at once.
+    // order; the four blocks' products interleave.
     const STRATA_GLOBAL bf16x8 *tp = gp(tab) + lane * 2;
-    bf16x8 bh = tp[0], bl = tp[1];
     const __bf16 *ahp = &hi[(n)*PITCH + c0 + 8 * kg];
     const __bf16 *alp = &lo[(n)*PITCH + c0 + 8 * kg];
-#pragma unroll 1
-    for (int i = 0; i < N; ++i) {
-        const int ni = i + 1 < N ? i + 1 : i;
-        const bf16x8 nbh = tp[ni * 128], nbl = tp[ni * 128 + 1];
+    auto row = [&](int i, const bf16x8 &bh, const bf16x8 &bl) __attribute__((always_inline)) {
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -1540,40 +1527,53 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
         for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b], bl, acc[b], 0, 0, 0);
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b], bh, acc[b], 0, 0, 0);
-        bh = nbh;
-        bl = nbl;
+    };
+    // Weight rows in pairs with two operand sets, so each row's B operands
+    // load a row ahead of their products with no copies between sets; the
+    // scheduling barriers keep each load ahead of the products it must
+    // overlap. Row 0's operands pass through an empty asm: with a load on
+    // both sides of the loop's phi, the compiler would turn the phi into
+    // one load at the top of each iteration, waited for right away.
+    bf16x8 b0h = tp[0], b0l = tp[1];
+    __asm__ volatile("" : "+v"(b0h), "+v"(b0l));
+    int i = 0;
+#pragma unroll 1
+    for (; i + 1 < N; i += 2) {
+        const bf16x8 b1h = tp[(i + 1) * 128], b1l = tp[(i + 1) * 128 + 1];
+        __builtin_amdgcn_sched_barrier(0);
+        row(i, b0h, b0l);
+        const int i2 = i + 2 < N ? i + 2 : N - 1;
+        b0h = tp[i2 * 128];
+        b0l = tp[i2 * 128 + 1];
+        __builtin_amdgcn_sched_barrier(0);
+        row(i + 1, b1h, b1l);
     }
+    if (i < N) row(i, b0h, b0l);  // the last row of an odd count
+}
 
+// A whole tile: inside the region and the raster's interior, with nothing
+// to recompute and row offsets within 32 bits.
+template <int R>
+__device__ __forceinline__ bool mfma_whole(const Region &rg, const DstView &d0, int64_t tx0, int64_t ty0, bool fix)
+{
+    return !fix && tx0 + kMfmaTile <= rg.ox + rg.w && ty0 + kMfmaTile <= rg.oy + rg.h && tx0 >= R &&
+           tx0 + kMfmaTile <= rg.W - R && ty0 >= R && ty0 + kMfmaTile <= rg.H - R &&
+           (int64_t)kMfmaTile * d0.stride < ((int64_t)1 << 29);
+}
+
+// Stores one wave's products from its registers: a whole tile without
+// conditions at 32-bit offsets; any other tile cell by cell, with the edge
+// ring and, where the window held a non-finite value (fix), the exact FP32
+// value.
+template <int R>
+__device__ __forceinline__ void mfma_store(const float *wts, const Region &rg, const SrcView &s, const DstView &d0,
+                                           int64_t tx0, int64_t ty0, int lane, int wave, bool fix,
+                                           const f32x4 (&acc)[4])
+{
+    const int n = lane & 15, kg = lane >> 4;
+    const int c0 = wave * 16;
     const int64_t x = tx0 + c0 + n;
-    const bool fix = nonfinite != 0;
-    // A whole tile inside the region and the interior, with nothing to
-    // recompute, stores without conditions at 32-bit offsets.
-    const bool whole = !fix && tx0 + kMfmaTile <= rg.ox + rg.w && ty0 + kMfmaTile <= rg.oy + rg.h && tx0 >= R &&
-                       tx0 + kMfmaTile <= rg.W - R && ty0 >= R && ty0 + kMfmaTile <= rg.H - R &&
-                       (int64_t)kMfmaTile * d0.stride < ((int64_t)1 << 29);
-    if (whole && (reinterpret_cast<uintptr_t>(d0.p) & 15u) == 0 && (d0.stride & 3) == 0) {
-        // Through LDS, so each thread stores 16 consecutive outputs of one
-        // row with four 16-byte stores (a row of 68 floats puts the 64
-        // lanes of each write in 32 distinct banks, two lanes a bank).
-        constexpr int OP = 68;
-        static_assert(kMfmaTile * OP * 4 <= 2 * LH * PITCH * 2, "the outputs fit the staging planes");
-        float *ot = reinterpret_cast<float *>(planes);
-        __syncthreads();  // every wave is done reading the planes
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) ot[(16 * b + 4 * kg + e) * OP + c0 + n] = acc[b][e];
-        __syncthreads();
-        const int r = tid >> 2, cq = 16 * (tid & 3);
-        char *ob = reinterpret_cast<char *>(d0.p + ((ty0 - rg.oy) * d0.stride + (tx0 - rg.ox)));
-        const uint32_t o0 = (uint32_t)r * ((uint32_t)d0.stride * 4u) + 4u * (uint32_t)cq;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            *reinterpret_cast<STRATA_GLOBAL f32x4 *>(gp(ob) + o0 + 16u * q) =
-                *reinterpret_cast<const f32x4 *>(ot + r * OP + cq + 4 * q);
-        return;
-    }
-    if (whole) {
+    if (mfma_whole<R>(rg, d0, tx0, ty0, fix)) {
         char *ob = reinterpret_cast<char *>(d0.p + ((ty0 - rg.oy) * d0.stride + (tx0 - rg.ox)));
         const uint32_t st4 = (uint32_t)d0.stride * 4u;
         const uint32_t o0 = (uint32_t)(4 * kg) * st4 + 4u * (uint32_t)(c0 + n);
@@ -1601,6 +1601,80 @@ __global__ __launch_bounds__(256) void correlate_mfma(const Params P, Region rg,
             }
             gp(d0.p)[(y - rg.oy) * d0.stride + (x - rg.ox)] = interior ? v : edge_nan();
         }
+}
+
+// correlate_mfma_pipe: a workgroup of eight waves stays resident and loops
+// over tiles, so the staging runs beside the products: waves
+// 4..7 stage the next tile into one of two LDS buffers while waves 0..3 run
+// the current tile's products from the other and store them, and one
+// barrier per tile hands the buffers over. Every thread of the workgroup
+// runs the same number of iterations (the count depends on the workgroup
+// only), so every barrier is reached by all of them. The grid is a
+// multiple of the XCD count; workgroup b serves XCD b % 8, which owns one
+// contiguous run of tiles in row-major order, and its workgroups take that
+// run in steps, so the tiles in flight on one XCD are neighbours.
+// Six waves per SIMD (three workgroups a CU, which their LDS allows at
+// every radius) bound the registers.
+template <int R>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void correlate_mfma_pipe(
+    const Params P, Region rg, SrcView s, DstView d0, const bf16x8 *__restrict__ tab)
+{
+    constexpr int N = 2 * R + 1;
+    constexpr int LH = kMfmaTile + 2 * R, PITCH = mfma_pitch<R>(), PLANES = 2 * LH * PITCH;
+    __shared__ __attribute__((aligned(16))) __bf16 planes[2 * PLANES];
+    __shared__ float wts[N * N];
+    // Per buffer: the iteration whose tile, staged there, held a
+    // non-finite value (-1: none yet).
+    __shared__ int nonfinite[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t gx = (uint32_t)((rg.w + kMfmaTile - 1) / kMfmaTile);
+    const uint32_t ntiles = gx * (uint32_t)((rg.h + kMfmaTile - 1) / kMfmaTile);
+    const uint32_t S = gridDim.x / kXcds, xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t per = ntiles / kXcds, extra = ntiles % kXcds;
+    const uint32_t first = xcd * per + min(xcd, extra) + slot, cnt = per + (xcd < extra ? 1u : 0u);
+    // This workgroup's tiles: first + k*S for k < K.
+    const int K = slot < cnt ? (int)((cnt - slot + S - 1) / S) : 0;
+    if (K == 0) return;  // the whole workgroup, before any barrier
+    auto origin = [&](int k, int64_t &tx0, int64_t &ty0) {
+        const uint32_t t = first + (uint32_t)k * S;
+        tx0 = rg.ox + (int64_t)(t % gx) * kMfmaTile;
+        ty0 = rg.oy + (int64_t)(t / gx) * kMfmaTile;
+    };
+    if (tid < 2) nonfinite[tid] = -1;
+    for (int i = tid; i < N * N; i += 512) wts[i] = P.taps[i];
+    __syncthreads();
+    const bool producer = wave >= 4;
+    if (producer) {
+        int64_t tx0, ty0;
+        origin(0, tx0, ty0);
+        if (mfma_stage<R>(s, tx0, ty0, tid - 256, planes, planes + LH * PITCH)) nonfinite[0] = 0;
+    }
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        if (producer) {
+            if (k + 1 < K) {
+                int64_t tx0, ty0;
+                origin(k + 1, tx0, ty0);
+                __bf16 *pl = planes + ((k + 1) & 1) * PLANES;
+                // The thread's staging offsets are derived anew each tile
+                // from an opaque copy of its index: hoisted out of the
+                // loop, they would stay live through it and spill.
+                int p = tid - 256;
+                __asm__ volatile("" : "+v"(p));
+                if (mfma_stage<R>(s, tx0, ty0, p, pl, pl + LH * PITCH)) nonfinite[(k + 1) & 1] = k + 1;
+            }
+        } else {
+            int64_t tx0, ty0;
+            origin(k, tx0, ty0);
+            const __bf16 *pl = planes + (k & 1) * PLANES;
+            int ln = lane;  // opaque, as the producers' index
+            __asm__ volatile("" : "+v"(ln));
+            f32x4 acc[4];
+            mfma_products<R>(pl, pl + LH * PITCH, tab, ln, wave, acc);
+            mfma_store<R>(wts, rg, s, d0, tx0, ty0, ln, wave, nonfinite[k & 1] == k, acc);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2484,6 +2558,16 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
     hipError_t e = scratch_alloc_async((void **)&tab, sizeof(bf16x8) * 2 * 64 * N, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mfma_toeplitz<R>, dim3(N), dim3(64), 0, st, P, tab);
+    // Resident workgroups per XCD: the device's CUs times the kernel's
+    // occupancy, over the XCDs.
+    int dev = 0, cus = 0, per_cu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, correlate_mfma_pipe<R>, 512, 0)) != hipSuccess) {
+        (void)hipFreeAsync(tab, st);
+        return e;
+    }
+    const int64_t resident = std::max<int64_t>(1, (int64_t)cus * std::max(per_cu, 1) / kXcds);
     const int64_t gx = cdiv(rg.w, kMfmaTile), step = std::max<int64_t>(1, kMaxTileBlocks / gx);
     for (int64_t yb = 0; yb < cdiv(rg.h, kMfmaTile); yb += step) {
         Region sub = rg;
@@ -2491,8 +2575,11 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
         sub.h = min(rg.h - yb * kMfmaTile, step * kMfmaTile);
         DstView e0 = d[0];
         e0.p += yb * kMfmaTile * e0.stride;
-        hipLaunchKernelGGL(correlate_mfma<R>, dim3((unsigned)(gx * cdiv(sub.h, kMfmaTile))), dim3(256), 0, st, P,
-                           sub, s, e0, tab);
+        // As many resident workgroups as the chip holds, in whole multiples
+        // of the XCD count, and no more than the tiles need.
+        const int64_t ntiles = gx * cdiv(sub.h, kMfmaTile);
+        const int64_t grid = std::max<int64_t>(kXcds, std::min<int64_t>(resident, cdiv(ntiles, kXcds)) * kXcds);
+        hipLaunchKernelGGL(correlate_mfma_pipe<R>, dim3((unsigned)grid), dim3(512), 0, st, P, sub, s, e0, tab);
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(tab, st);
