@@ -1293,6 +1293,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaTile = 64;
+constexpr uint32_t kMfmaStrip = 16;  // tiles per strip of correlate_mfma_pipe's order
 
 // v = h + l with h = bf16(v) and l = bf16(v - h), for finite v. Finite
 // values above the largest bf16 (0x7f7f) would round to infinity; their
@@ -1611,8 +1612,7 @@ __device__ __forceinline__ void mfma_store(const float *wts, const Region &rg, c
 // runs the same number of iterations (the count depends on the workgroup
 // only), so every barrier is reached by all of them. The grid is a
 // multiple of the XCD count; workgroup b serves XCD b % 8, which owns one
-// contiguous run of tiles in row-major order, and its workgroups take that
-// run in steps, so the tiles in flight on one XCD are neighbours.
+// band of tile rows, and the band's workgroups take its tiles in steps.
 // Six waves per SIMD (three workgroups a CU, which their LDS allows at
 // every radius) bound the registers.
 template <int R>
@@ -1628,17 +1628,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void c
     __shared__ int nonfinite[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t gx = (uint32_t)((rg.w + kMfmaTile - 1) / kMfmaTile);
-    const uint32_t ntiles = gx * (uint32_t)((rg.h + kMfmaTile - 1) / kMfmaTile);
+    const uint32_t gy = (uint32_t)((rg.h + kMfmaTile - 1) / kMfmaTile);
     const uint32_t S = gridDim.x / kXcds, xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
-    const uint32_t per = ntiles / kXcds, extra = ntiles % kXcds;
-    const uint32_t first = xcd * per + min(xcd, extra) + slot, cnt = per + (xcd < extra ? 1u : 0u);
-    // This workgroup's tiles: first + k*S for k < K.
+    // The XCD's band of tile rows, taken in strips kMfmaStrip tiles wide,
+    // each strip row by row: the S tiles in flight on the XCD form a block
+    // about kMfmaStrip wide, so a tile's neighbours on every side, whose
+    // windows share its halo, are staged close to it in time and the
+    // shared lines come from the XCD's L2.
+    const uint32_t per = gy / kXcds, extra = gy % kXcds;
+    const uint32_t ry0 = xcd * per + min(xcd, extra), nrows = per + (xcd < extra ? 1u : 0u);
+    const uint32_t cnt = nrows * gx;
+    // This workgroup's tiles: XCD-local numbers slot + k*S for k < K.
     const int K = slot < cnt ? (int)((cnt - slot + S - 1) / S) : 0;
     if (K == 0) return;  // the whole workgroup, before any barrier
     auto origin = [&](int k, int64_t &tx0, int64_t &ty0) {
-        const uint32_t t = first + (uint32_t)k * S;
-        tx0 = rg.ox + (int64_t)(t % gx) * kMfmaTile;
-        ty0 = rg.oy + (int64_t)(t / gx) * kMfmaTile;
+        const uint32_t j = slot + (uint32_t)k * S;
+        const uint32_t strip = j / (kMfmaStrip * nrows), jj = j - strip * kMfmaStrip * nrows;
+        const uint32_t ws = min(kMfmaStrip, gx - strip * kMfmaStrip);
+        tx0 = rg.ox + (int64_t)(strip * kMfmaStrip + jj % ws) * kMfmaTile;
+        ty0 = rg.oy + (int64_t)(ry0 + jj / ws) * kMfmaTile;
     };
     if (tid < 2) nonfinite[tid] = -1;
     for (int i = tid; i < N * N; i += 512) wts[i] = P.taps[i];
@@ -2577,8 +2585,8 @@ static hipError_t run_mfma_r(const Params &P, const Region &rg, const SrcView &s
         e0.p += yb * kMfmaTile * e0.stride;
         // As many resident workgroups as the chip holds, in whole multiples
         // of the XCD count, and no more than the tiles need.
-        const int64_t ntiles = gx * cdiv(sub.h, kMfmaTile);
-        const int64_t grid = std::max<int64_t>(kXcds, std::min<int64_t>(resident, cdiv(ntiles, kXcds)) * kXcds);
+        const int64_t xcd_tiles = gx * cdiv(cdiv(sub.h, kMfmaTile), kXcds);  // the largest band's tiles
+        const int64_t grid = std::max<int64_t>(kXcds, std::min<int64_t>(resident, xcd_tiles) * kXcds);
         hipLaunchKernelGGL(correlate_mfma_pipe<R>, dim3((unsigned)grid), dim3(512), 0, st, P, sub, s, e0, tab);
     }
     e = hipGetLastError();
