@@ -859,6 +859,9 @@ def focal_bench(args, dev, log):
         kmf = focal.newWeightsKernel(focal.WeightsOptions(r, w), False, mfma=True)
         ops.append((f"Correlate r={r} MFMA split-BF16x3", kmf, 2 * n * n - 1))
         errs[r] = (k32, kmf)
+    # Executed MFMA flops per cell: per weight row, three 16x16x32 products
+    # (16*16*32*2 flops each) per 16x16 output block.
+    mfma_flops = {f"Correlate r={r} MFMA split-BF16x3": (2 * r + 1) * 3 * 16 * 16 * 32 * 2 // 256 for r in (4, 8)}
     for r in (1, 2, 4, 8):
         g = focal.Gaussian(r, r / 2)
         ops.append((f"CorrelateSeparable Gaussian r={r}", focal.newSeparableKernel(focal.SeparableOptions(r, g, g)),
@@ -887,6 +890,9 @@ def focal_bench(args, dev, log):
                             "data_kernel_ms": round(kms, 4),
                             "hbm_frac": round(8.25 * cells / el / 1e9 / HBM_PEAK_GBS, 4),
                             "fp32_tflops": round(flops * cells / (kms * 1e-3) / 1e12, 2)}
+        if name in mfma_flops:
+            res["ops"][name]["mfma_executed_flops_per_cell"] = mfma_flops[name]
+            res["ops"][name]["mfma_executed_tflops"] = round(mfma_flops[name] * cells / (kms * 1e-3) / 1e12, 1)
         log(f"bench: focal {name}: {res['ops'][name]}")
     # Precision of both Correlate paths against the FP64 correlation (on the
     # device), over every interior cell.
@@ -902,7 +908,38 @@ def focal_bench(args, dev, log):
             log(f"bench: focal error r={r} {label}: {res['error_vs_f64'][f'r={r} {label}']}")
     del src, dst, data, valid
     torch.cuda.empty_cache()
+    res["bf16_gemm"] = bf16_gemm_rate(dev, log)
     return res
+
+
+def bf16_gemm_rate(dev, log):
+    """The dense bf16 rate the vendor library reaches on this GPU: an
+    8192x8192x8192 torch.mm of contiguous bf16 operands (hipBLASLt), the
+    best of five timed runs after a warm-up."""
+    import torch
+
+    n = 8192
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    a = torch.randn(n, n, generator=g, device=dev, dtype=torch.float32).to(torch.bfloat16).contiguous()
+    b = torch.randn(n, n, generator=g, device=dev, dtype=torch.float32).to(torch.bfloat16).contiguous()
+    c = torch.mm(a, b)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            torch.mm(a, b, out=c)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 10)
+    out = {"shape": [n, n, n], "ms": round(best, 4), "tflops": round(2 * n ** 3 / (best * 1e-3) / 1e12, 1),
+           "what": "torch.mm bf16 (hipBLASLt) on contiguous operands, best of 5 runs of 10"}
+    log(f"bench: bf16 GEMM {out}")
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving_bench(args, dem, N, log):
