@@ -32,8 +32,6 @@ struct strata_batch {
     size_t dev_bytes = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipStream_t side = nullptr;  // the focal list's stream
-    hipEvent_t ev[2] = {nullptr, nullptr};
     int rows = 64, blocks = 0, blocks_per_cu = 0, cus = 1;
     bool graph = false;
     hipGraph_t g = nullptr;
@@ -45,9 +43,6 @@ struct strata_batch {
         if (gx) (void)hipGraphExecDestroy(gx);
         if (g) (void)hipGraphDestroy(g);
         if (dev) (void)hipFree(dev);
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (side) (void)hipStreamDestroy(side);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -285,7 +280,7 @@ hipError_t enqueue(strata_batch *b)
     };
     // Replays reuse the uploaded lists: the counters are cleared by a kernel.
     const BatchLaunch L = launch_view(b->layout, b->dev, cap(BF_HORN), cap(BF_FOCAL), true);
-    return launch_batch(L, b->stream, b->side, b->ev);
+    return launch_batch(L, b->stream);
 }
 
 int set_shape(strata_batch *b, int rows, int blocks_per_cu, bool graph)
@@ -389,15 +384,6 @@ int strata_batch_create(const strata_task *tasks, int ntasks, const strata_batch
         }
         b->own_stream = true;
     }
-    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return fail(STRATA_EDEVICE, "strata: cannot create a stream");
-    }
-    for (hipEvent_t &e : b->ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            delete b;
-            return fail(STRATA_EDEVICE, "strata: cannot create an event");
-        }
     const int rc = set_shape(b, opts && opts->rows ? opts->rows : 64, opts ? opts->blocks_per_cu : 0,
                              opts && opts->graph);
     if (rc) {

@@ -2788,33 +2788,24 @@ void batch_units(const BatchOp &op, BatchTask &t, int rows)
     if (t.vgroup) t.vwaves_x = 1;
 }
 
-hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side, hipEvent_t *ev)
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st)
 {
-    hipError_t e;
     if (L.clear) {
         hipLaunchKernelGGL(batch_clear, dim3(1), dim3(64), 0, st, L.counters);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    const bool focal = L.nunits[BF_FOCAL] > 0;
-    const bool fork = focal && side && ev;
-    if (fork) {
-        // The focal list runs on a side stream, joined back before the
-        // batch counts as done (also inside a graph capture).
-        if ((e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side, ev[0], 0)) != hipSuccess) return e;
-    }
-    if (focal)
-        hipLaunchKernelGGL(batch_focal_kernel, dim3((unsigned)L.blocks_focal), dim3(256), 0, fork ? side : st,
-                           L.hdr, L.tasks[BF_FOCAL], L.ops, L.counters + kCounterStride);
+    // The focal list, then the wave lists, one after the other on st. With
+    // the focal kernel on a second stream beside the wave kernel, the two
+    // shared the chip and the bandwidth, and the mixed 4096-tile batch took
+    // 7.52 ms against 7.24 ms one after the other.
+    if (L.nunits[BF_FOCAL] > 0)
+        hipLaunchKernelGGL(batch_focal_kernel, dim3((unsigned)L.blocks_focal), dim3(256), 0, st, L.hdr,
+                           L.tasks[BF_FOCAL], L.ops, L.counters + kCounterStride);
     if (L.nunits[BF_HORN] + L.nunits[BF_VALIDITY] > 0)
         hipLaunchKernelGGL(batch_wave_kernel, dim3((unsigned)L.blocks_wave), dim3(256), 0, st, L.hdr,
                            L.tasks[BF_HORN], L.tasks[BF_VALIDITY], L.ops, L.counters);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (fork) {
-        if ((e = hipEventRecord(ev[1], side)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(st, ev[1], 0)) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 int batch_blocks_per_cu(int family)

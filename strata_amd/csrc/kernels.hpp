@@ -295,9 +295,9 @@ struct BatchLaunch {
 // blocks_x (data units across) and vwaves_x.
 void batch_units(const BatchOp &op, BatchTask &t, int rows);
 
-// Runs the lists' persistent kernels on st; with a side stream and two
-// events, the focal list runs on the side stream, joined back into st.
-hipError_t launch_batch(const BatchLaunch &L, hipStream_t st, hipStream_t side, hipEvent_t *ev);
+// Runs the lists' persistent kernels on st: the focal list, then the wave
+// lists.
+hipError_t launch_batch(const BatchLaunch &L, hipStream_t st);
 
 // Workgroups of a list's kernel resident per CU (occupancy), for sizing.
 int batch_blocks_per_cu(int family);

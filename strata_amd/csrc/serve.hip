@@ -201,7 +201,7 @@ bool strata_server::launch(Slot &s, std::vector<int64_t> &batch, std::unique_loc
 
     lk.unlock();
     hipError_t e = hipMemcpyAsync(s.dev, s.host, L.bytes, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess) e = launch_batch(view, s.stream, nullptr, nullptr);
+    if (e == hipSuccess) e = launch_batch(view, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
     const double launched = seconds(Clock::now());
     lk.lock();
