@@ -221,6 +221,19 @@ ListLayout write_lists(const BatchOp *ops, int nops, const BatchTask *tasks, con
         L.h[f].ntasks = (int64_t)count[f];
         L.h[f].nunits = u[f];
     }
+    // One focal kind and radius in the whole list: its own kernel.
+    for (size_t i = 0; i < n; ++i) {
+        const BatchOp &o = ops[tasks[i].op];
+        if (o.kind == BK_HORN) continue;
+        const int code = batch_focal_one_code(o.kind, o.radius);
+        if (L.focal_one == -1) {
+            L.focal_one = code;
+        } else if (L.focal_one != code) {
+            L.focal_one = -2;  // mixed
+            break;
+        }
+    }
+    if (L.focal_one < 0) L.focal_one = -1;
     memcpy(out, L.h, sizeof L.h);
     memset(out + L.counters, 0, 256 * kBatchFamilies);
     return L;
@@ -240,6 +253,7 @@ BatchLaunch launch_view(const ListLayout &L, char *dev, int max_wave_blocks, int
     b.blocks_focal = (int)std::max<int64_t>(1, std::min<int64_t>(max_focal_blocks, L.h[BF_FOCAL].nunits));
     b.counters = reinterpret_cast<unsigned *>(dev + L.counters);
     b.clear = clear;
+    b.focal_one = L.focal_one;
     return b;
 }
 
@@ -275,7 +289,7 @@ int build_list(strata_batch *b, int rows)
 hipError_t enqueue(strata_batch *b)
 {
     auto cap = [&](int family) {
-        const int occ = batch_blocks_per_cu(family);
+        const int occ = batch_blocks_per_cu(family, b->layout.focal_one);
         return b->cus * (b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, occ) : occ);
     };
     // Replays reuse the uploaded lists: the counters are cleared by a kernel.

@@ -30,6 +30,7 @@ BatchTask make_task(const strata_task &t, int32_t op);
 struct ListLayout {
     size_t ops = 0, tasks[kBatchFamilies] = {0, 0, 0}, counters = 0, bytes = 0;
     BatchHeader h[kBatchFamilies] = {};
+    int focal_one = -1;  // every focal task's kind and radius (batch_focal_one_code), or -1
 };
 // Bytes of the buffer for nops ops and ntasks tasks (an upper bound).
 size_t list_bytes(int nops, size_t ntasks);

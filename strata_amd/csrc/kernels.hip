@@ -883,7 +883,10 @@ constexpr int focal_lds_floats()
 // counts, several LDS reads in flight together. Inlined beside the other
 // kinds inside the batch's tile loop, that form needs several times the
 // registers; the batch stages row by row and keeps the column loop rolled.
-template <int KIND, int R, class TAPS, bool UNROLL = true>
+// IN_LOOP (a caller that runs tiles in a loop): the thread index passes
+// through an empty asm, so the staging offsets derived from it are formed
+// per tile instead of hoisted out of the loop and held in registers.
+template <int KIND, int R, class TAPS, bool UNROLL = true, bool IN_LOOP = false>
 __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps, const Region &rg,
                                             const SrcView &s, const DstView &d0, const DstView &d1, int64_t tx0,
                                             int64_t ty0, float *lds, const DstView *dset = nullptr)
@@ -895,7 +898,8 @@ __device__ __forceinline__ void focal_block(const FocalArgs &P, const TAPS &taps
     float(*tile)[LW] = reinterpret_cast<float(*)[LW]>(lds);
     float(*colv)[CY][LW] = reinterpret_cast<float(*)[CY][LW]>(lds + LH * LW);
 
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    if constexpr (IN_LOOP) __asm__ volatile("" : "+v"(tid));
     // Staging. Both forms issue every load of the thread before any LDS
     // write. UNROLL (a kernel per op): cells in thread order, unchecked
     // loads when the staged window lies inside the buffer. Otherwise (the
@@ -2414,6 +2418,56 @@ __global__ __launch_bounds__(256) void batch_focal_kernel(const BatchHeader *__r
     }
 }
 
+// The focal list when every focal task has the same kind and radius: each
+// tile is one focal_block of that op alone, with the per-op kernels'
+// cell-by-cell staging (every load of the thread in flight at 32-bit
+// offsets), which the kernel that inlines every kind cannot afford in
+// registers.
+template <int KIND, int R>
+__global__ __launch_bounds__(256) void batch_focal_one(const BatchHeader *__restrict__ hdr,
+                                                       const BatchTask *__restrict__ tasks,
+                                                       const BatchOp *__restrict__ ops, unsigned *counter)
+{
+    __shared__ float lds[kBatchLdsFloats];
+    static_assert(focal_lds_floats<KIND, R>() <= kBatchLdsFloats, "the tile fits the batch's LDS");
+    __shared__ unsigned claimed;
+    const int64_t ntasks = hdr[BF_FOCAL].ntasks, nunits = hdr[BF_FOCAL].nunits;
+    int64_t ti = 0, tu0 = 0, tu1 = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(counter, 1u);
+    __syncthreads();
+    int64_t u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+    while (u < nunits) {
+        unsigned next = 0u;
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);  // consumed after this unit
+        if (u < tu0 || u >= tu1) {
+            ti = batch_find(tasks, ntasks, u);
+            tu0 = tasks[ti].u0;
+            tu1 = tasks[ti].u1;
+        }
+        const BatchTask &t = tasks[ti];
+        const BatchOp &op = ops[t.op];
+        FocalArgs a;
+        a.op = op.op;
+        a.flags = op.flags;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) a.k[i] = op.k[i];
+        const PtrTaps taps{nullptr};  // box kinds read no taps
+        const int64_t lu = u - tu0;
+        const int64_t tx0 = t.rg.ox + (lu % t.blocks_x) * kTileX;
+        const int64_t ty = t.rg.oy + (lu / t.blocks_x) * t.rows;
+#pragma unroll 1
+        for (int k = 0; k < kFocalTilesPerUnit; ++k) {
+            const int64_t ty0 = ty + k * kTileY;
+            if (ty0 >= t.rg.oy + t.rg.h) break;
+            focal_block<KIND, R, PtrTaps, true, true>(a, taps, t.rg, t.s, t.d, t.d, tx0, ty0, lds);
+        }
+        __syncthreads();  // the unit is done (and the LDS free) on every wave
+        if (threadIdx.x == 0) claimed = next;
+        __syncthreads();
+        u = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(claimed);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -2799,21 +2853,60 @@ hipError_t launch_batch(const BatchLaunch &L, hipStream_t st)
     // the focal kernel on a second stream beside the wave kernel, the two
     // shared the chip and the bandwidth, and the mixed 4096-tile batch took
     // 7.52 ms against 7.24 ms one after the other.
-    if (L.nunits[BF_FOCAL] > 0)
-        hipLaunchKernelGGL(batch_focal_kernel, dim3((unsigned)L.blocks_focal), dim3(256), 0, st, L.hdr,
-                           L.tasks[BF_FOCAL], L.ops, L.counters + kCounterStride);
+    if (L.nunits[BF_FOCAL] > 0) {
+        const dim3 g((unsigned)L.blocks_focal);
+        const BatchTask *ft = L.tasks[BF_FOCAL];
+        unsigned *fc = L.counters + kCounterStride;
+        switch (L.focal_one) {
+#define STRATA_FOCAL_ONE(K, KIND, R)                                                                          \
+    case batch_focal_one_code(K, R):                                                                          \
+        hipLaunchKernelGGL((batch_focal_one<KIND, R>), g, dim3(256), 0, st, L.hdr, ft, L.ops, fc);          \
+        break;
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 1)
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 2)
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 3)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 1)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 2)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 3)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 1)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 2)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 3)
+#undef STRATA_FOCAL_ONE
+        default:
+            hipLaunchKernelGGL(batch_focal_kernel, g, dim3(256), 0, st, L.hdr, ft, L.ops, fc);
+        }
+    }
     if (L.nunits[BF_HORN] + L.nunits[BF_VALIDITY] > 0)
         hipLaunchKernelGGL(batch_wave_kernel, dim3((unsigned)L.blocks_wave), dim3(256), 0, st, L.hdr,
                            L.tasks[BF_HORN], L.tasks[BF_VALIDITY], L.ops, L.counters);
     return hipGetLastError();
 }
 
-int batch_blocks_per_cu(int family)
+int batch_blocks_per_cu(int family, int focal_one)
 {
     int n = 0;
-    const hipError_t e =
-        family == BF_FOCAL ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_focal_kernel, 256, 0)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_wave_kernel, 256, 0);
+    hipError_t e = hipSuccess;
+    if (family != BF_FOCAL) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_wave_kernel, 256, 0);
+    } else {
+        switch (focal_one) {
+#define STRATA_FOCAL_ONE(K, KIND, R)                                                                          \
+    case batch_focal_one_code(K, R):                                                                          \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_focal_one<KIND, R>, 256, 0);               \
+        break;
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 1)
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 2)
+            STRATA_FOCAL_ONE(BK_MEAN, FK_MEAN, 3)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 1)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 2)
+            STRATA_FOCAL_ONE(BK_MIN, FK_MIN, 3)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 1)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 2)
+            STRATA_FOCAL_ONE(BK_MAX, FK_MAX, 3)
+#undef STRATA_FOCAL_ONE
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_focal_kernel, 256, 0);
+        }
+    }
     return (e != hipSuccess || n < 1) ? 1 : n;
 }
 

@@ -289,7 +289,14 @@ struct BatchLaunch {
     int64_t nunits[kBatchFamilies];
     int blocks_wave, blocks_focal;
     bool clear;  // zero the counters first (they are not uploaded zeroed)
+    // When every focal task has one kind and radius: 4 * kind + radius
+    // (batch_focal_one_code), and the focal list runs through a kernel
+    // built for that op alone; -1 otherwise.
+    int focal_one;
 };
+
+// The code of a focal kind (BK_MEAN, BK_MIN, BK_MAX) and radius (1..3).
+constexpr int batch_focal_one_code(int kind, int radius) { return 4 * kind + radius; }
 
 // Fills a task's unit shape for `rows` rows per radius-1 unit: rows,
 // blocks_x (data units across) and vwaves_x.
@@ -300,6 +307,6 @@ void batch_units(const BatchOp &op, BatchTask &t, int rows);
 hipError_t launch_batch(const BatchLaunch &L, hipStream_t st);
 
 // Workgroups of a list's kernel resident per CU (occupancy), for sizing.
-int batch_blocks_per_cu(int family);
+int batch_blocks_per_cu(int family, int focal_one = -1);
 
 } // namespace strata

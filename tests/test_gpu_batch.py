@@ -41,12 +41,15 @@ RECLASS = transfer.newReclassKernel("t", [1.2, 1.6, 2.2, 3], [1, 2, 3, 4, 5])
 CHAINS = [(), (LOOKUP,), (RECLASS,), (LOOKUP, RECLASS)]
 
 
-def build(seed: int, n: int):
+def build(seed: int, n: int, focal_only=None):
     """A scene of n tasks: windows of two source parents (one masked at a
     mid-word bit base, one unmasked), each task writing a window of its own
-    output parent."""
+    output parent. focal_only: the one focal op (by name) among the ops."""
     rng = np.random.default_rng(seed)
     ops = stencil_ops()
+    if focal_only is not None:
+        ops = [(nm, k) for nm, k in ops if k.radius == 1 and nm[:4] in ("slop", "aspe", "hill", "curv")
+               or nm == focal_only]
     sc = Scene()
     SW, SH = 1300, 900
     sc.add_parent("srcm", rng, SW, SH, masked=True, valid_base=29)
@@ -113,6 +116,22 @@ def test_batch_matches_separate_calls(gpu, seed, n):
         info = b.info()
         assert info["tasks"] == n
         assert info["cells"] == sum(t.src.Width * t.src.Height for t in dt)
+
+
+@pytest.mark.parametrize("focal", ["mean3", "min1", "max2"])
+def test_batch_one_focal_kind_matches_separate_calls(gpu, focal):
+    """Every focal task of one kind and radius, beside the terrain ops: the
+    batch runs its focal list through the kernel built for that op alone."""
+    import torch
+    sc, tasks = build(7, 36, focal_only=focal)
+    assert {nm for nm, k, _ in tasks if nm not in [o[0] for o in stencil_ops()[:8]]} == {focal}
+    want = expected(sc, tasks)
+    dt, dbufs = device_tasks(sc, tasks, gpu)
+    with Batch(dt) as b:
+        b.run()
+        b.sync()
+        torch.cuda.synchronize()
+        assert_same_parents(host_copy(dbufs), want, f"batch of {focal}")
 
 
 @pytest.mark.parametrize("rows,bpc", [(4, 1), (16, 2), (128, 0), (1000, 0)])
